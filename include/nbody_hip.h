@@ -1,0 +1,183 @@
+/*
+ * nbody_hip.h -- C ABI of the MI355X (gfx950) Hermite-4 force backend.
+ *
+ * Plain C types only: host pointers and sizes, an opaque context, int status
+ * codes.  libnbody_hip.so exports exactly the functions declared here.  Each
+ * entry point names the reference interface it replaces
+ * (jlalmerol/N-Body-Code, file:line).
+ *
+ * Body state is struct-of-arrays, one double[n] per field, in the order of the
+ * reference's force_calculation arguments (include/kernel.h:15-19):
+ *   positions x, y, z; mass m (the reference's h_par_w); velocities vx, vy, vz.
+ * Forces are acceleration ax, ay, az and jerk jx, jy, jz (h_acc_*, h_adot_*).
+ *
+ * Errors: every int-returning call returns NBH_OK (0) or an NBH_ERR_* code;
+ * nbh_last_error() then describes the most recent failure on the calling
+ * thread.  A context is single-threaded: calls on one context must not
+ * overlap.  Calls that take host arrays are synchronous (outputs valid on
+ * return), matching the reference's blocking force_calculation.
+ */
+#ifndef NBODY_HIP_H
+#define NBODY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nbh_status {
+    NBH_OK = 0,
+    NBH_ERR_ARG = 1,       /* bad argument (n, pointer, config field) */
+    NBH_ERR_HIP = 2,       /* HIP runtime error (no device, launch failure) */
+    NBH_ERR_ALLOC = 3,     /* device or host allocation failed */
+    NBH_ERR_STATE = 4,     /* call out of order (e.g. step before upload) */
+    NBH_ERR_NONFINITE = 5  /* result holds inf/NaN (e.g. eps2 = 0, coincident bodies) */
+};
+
+enum nbh_precision {
+    NBH_FP64 = 0,  /* plain FP64 pair arithmetic: the drop-in */
+    NBH_MIXED = 1  /* FP64 x_j - x_i rounded to FP32; FP32 r^2, rsqrt, jerk;
+                      two-sum compensated FP32 accumulation */
+};
+
+typedef struct nbh_ctx nbh_ctx;
+
+typedef struct nbh_config {
+    int device;     /* HIP device ordinal */
+    int precision;  /* enum nbh_precision */
+    double eps2;    /* Plummer softening squared; the reference hard-codes
+                       EPS2 = 1e-7 (include/kernel.h:13) */
+    int block;      /* i-bodies (= threads) per workgroup and j-tile length:
+                       64, 128 or 256; 0 = pick by size */
+    int jsplit;     /* j-range split across workgroups; 0 = pick by size */
+} nbh_config;
+
+/* Description of the last failure on this thread ("" if none). */
+const char *nbh_last_error(void);
+
+/* ABI version of this library (major*100 + minor). */
+int nbh_version(void);
+
+/* Opens a context on cfg->device with its own stream.  Replaces the
+ * per-call device open/buffer/kernel set-up of the TT dispatch
+ * (src/kernel.cpp:54-243): here it happens once. */
+int nbh_create(const nbh_config *cfg, nbh_ctx **out);
+int nbh_destroy(nbh_ctx *ctx);
+
+/* Changes softening / precision / tiling of an existing context. */
+int nbh_configure(nbh_ctx *ctx, const nbh_config *cfg);
+
+/* Drop-in force evaluation: acceleration and jerk of all n bodies from all
+ * n bodies, self-pair excluded.  Replaces force_calculation
+ * (include/kernel.h:15-19; CPU FP64 src/kernel.cpp:1164-1224, _F32
+ * src/kernel.cpp:1101-1163, TT dispatch src/kernel.cpp:47-398).  Host
+ * arrays; outputs fully overwritten; synchronous.  kernel_ms (may be NULL)
+ * receives the device time of the pair kernels, the analogue of
+ * Sys::kernel_time (include/utils.h:42).  Returns NBH_ERR_NONFINITE if an
+ * output is inf/NaN (outputs are still written). */
+int nbh_force(nbh_ctx *ctx, int n,
+              const double *x, const double *y, const double *z,
+              const double *m, const double *vx, const double *vy,
+              const double *vz,
+              double *ax, double *ay, double *az,
+              double *jx, double *jy, double *jz, double *kernel_ms);
+
+/* Same evaluation on device-resident arrays (HBM pointers), enqueued on
+ * `stream` (a hipStream_t; NULL = the context's own stream), asynchronous.
+ * No counterpart in the reference (its arrays are host memory); it lets a
+ * caller that already holds device buffers skip the copies. */
+int nbh_force_device(nbh_ctx *ctx, int n,
+                     const double *x, const double *y, const double *z,
+                     const double *m, const double *vx, const double *vy,
+                     const double *vz,
+                     double *ax, double *ay, double *az,
+                     double *jx, double *jy, double *jz, void *stream);
+
+/* ---- device-resident integrator (NBody, include/nbody.h:24-59) ------ */
+
+/* Copies the state to HBM and evaluates the initial force
+ * (NBody::cycle's first force_calculation, src/nbody.cpp:310). */
+int nbh_upload(nbh_ctx *ctx, int n,
+               const double *x, const double *y, const double *z,
+               const double *m, const double *vx, const double *vy,
+               const double *vz);
+
+/* NBody::initialdt (src/nbody.cpp:131-150): min over bodies of
+ * eta0*|a|/|jerk| from the resident forces. */
+int nbh_initial_dt(nbh_ctx *ctx, double eta0, double *dt);
+
+/* Time-step control.  adaptive = 0: every step uses dt.  adaptive = 1: the
+ * first step uses dt and each later step the Aarseth value of
+ * NBody::updatedt (src/nbody.cpp:198-222, parameter eta). */
+int nbh_set_dt(nbh_ctx *ctx, double dt, int adaptive, double eta);
+int nbh_get_dt(nbh_ctx *ctx, double *dt);
+
+/* nsteps shared-step Hermite steps, NBody::timestepping
+ * (src/nbody.cpp:224-289): predict (:152-163), force at the predicted state
+ * (:232), snap/crackle + correct (:165-196), [updatedt (:261)], swap
+ * (:263-288).  use_graph != 0 replays the step sequence from a captured
+ * hipGraph.  Synchronous. */
+int nbh_step(nbh_ctx *ctx, int nsteps, int use_graph);
+
+/* Copies the resident state and current forces back; any pointer may be
+ * NULL to skip that array. */
+int nbh_download(nbh_ctx *ctx,
+                 double *x, double *y, double *z, double *m,
+                 double *vx, double *vy, double *vz,
+                 double *ax, double *ay, double *az,
+                 double *jx, double *jy, double *jz);
+
+/* Energy of the resident state in FP64.  compat = 0: kinetic and the
+ * physical softened potential -sum_{i<j} m_i m_j / sqrt(r^2 + eps2).
+ * compat = 1: compute_kinetic / compute_potential exactly as the reference
+ * defines them (src/utils.cpp:125-153: weight m_j*m_j, eps2 = 1e-7). */
+int nbh_energy(nbh_ctx *ctx, int compat, double *kinetic, double *potential);
+
+/* The same on host arrays (copied to the device for the evaluation). */
+int nbh_energy_host(nbh_ctx *ctx, int n,
+                    const double *x, const double *y, const double *z,
+                    const double *m, const double *vx, const double *vy,
+                    const double *vz, int compat,
+                    double *kinetic, double *potential);
+
+/* Per-kernel timing.  With profiling on, every pair-kernel launch is
+ * bracketed by HIP events on the stream it runs on; nbh_timing sums them.
+ * step_ms / steps cover nbh_step calls (host wall clock, synchronous). */
+int nbh_set_profiling(nbh_ctx *ctx, int on);
+int nbh_timing(nbh_ctx *ctx, double *force_ms, int64_t *force_launches,
+               double *step_ms, int64_t *steps);
+int nbh_reset_timing(nbh_ctx *ctx);
+
+/* Launch shape actually used for n bodies: block, jsplit, workgroups. */
+int nbh_launch_shape(nbh_ctx *ctx, int n, int *block, int *jsplit,
+                     int *workgroups);
+
+/* Device properties: compute units and wave size. */
+int nbh_device_info(int device, int *compute_units, int *wave_size,
+                    char *name, int name_len);
+
+/* ---- initial conditions (host code, no device needed) --------------- */
+
+/* Equal-mass Plummer sphere in Henon units (G = M = 1, E = -1/4):
+ * Aarseth-Henon-Wielen radii by inverse CDF with the mass fraction
+ * truncated at 0.999, velocities by von Neumann rejection, centre-of-mass
+ * frame.  PRNG: std::mt19937_64(seed), u = (draw >> 11) * 2^-53.
+ * No counterpart in the reference (its only generator is NBody::initialize,
+ * below); used for the benchmark configurations. */
+int nbh_plummer(int n, uint64_t seed,
+                double *x, double *y, double *z, double *m,
+                double *vx, double *vy, double *vz);
+
+/* NBody::initialize (src/nbody.cpp:69-114): the reference CLI's initial
+ * conditions (std::mt19937(42), uniform sphere of radius rmax, uniform
+ * velocity cube, power-law masses, virial rescale), bit-identical. */
+int nbh_reference_ic(int n, double vmax, double rmax, double mmin,
+                     double mmax, double *x, double *y, double *z, double *m,
+                     double *vx, double *vy, double *vz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_HIP_H */

@@ -1,0 +1,655 @@
+// nbh_api.hip -- the C ABI of include/nbody_hip.h: contexts, device
+// buffers, the drop-in force call and the device-resident integrator.
+//
+// A context owns one HIP stream (default priority) on one device and keeps
+// its device buffers across calls, growing them when n grows; the
+// reference's TT dispatch instead opened the device and built buffers on
+// every force_calculation (src/kernel.cpp:54-138, :397).
+#include "nbh_internal.h"
+#include "../../include/nbody_hip.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define NBH_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return fail(NBH_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grows to at least `need` bytes (contents not preserved).
+    hipError_t reserve(size_t need)
+    {
+        if (need <= bytes) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, need);
+        if (e == hipSuccess) bytes = need;
+        return e;
+    }
+    double* d() const { return static_cast<double*>(p); }
+};
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace
+
+struct nbh_ctx {
+    nbh_config cfg{};
+    int cus = 0;
+    hipStream_t stream = nullptr;
+
+    // transient buffers of the host-array calls
+    DevBuf in;       // 7n
+    DevBuf out;      // 6n
+    DevBuf partial;  // 6 * jsplit * n
+    DevBuf flag;     // int
+    DevBuf scal;     // scalars: [0] dt, [1] dt_next bits, [2..3] energy, [4..] spare
+
+    // resident state
+    int n = 0;
+    DevBuf state;  // 7n: x y z m vx vy vz
+    DevBuf f0;     // 6n: forces at the current state
+    DevBuf f1;     // 6n: forces at the predicted state
+    DevBuf ework;  // energy block sums
+    bool adaptive = false;
+    double eta = 1e-2;
+    bool dt_set = false;
+
+    // graph of the step sequence
+    hipGraphExec_t gexec = nullptr;
+    int g_steps = 0;
+    int g_n = -1;
+    bool g_adaptive = false;
+
+    // profiling
+    bool profiling = false;
+    std::vector<EventPair> ev_free, ev_pending;
+    double force_ms = 0.0;
+    int64_t force_launches = 0;
+    double step_ms = 0.0;
+    int64_t steps = 0;
+};
+
+namespace {
+
+nbh::Shape pick_shape(const nbh_ctx* c, int n)
+{
+    nbh::Shape s{};
+    int block = c->cfg.block;
+    if (block == 0) block = n >= 131072 ? 256 : (n >= 16384 ? 128 : 64);
+    s.block = block;
+    const int iblocks = (n + block - 1) / block;
+    const int jtiles = (n + block - 1) / block;
+    int split = c->cfg.jsplit;
+    if (split <= 0) {
+        // aim for about 8 workgroups per CU
+        const int want = 8 * (c->cus > 0 ? c->cus : 256);
+        split = (want + iblocks - 1) / iblocks;
+    }
+    if (split > jtiles) split = jtiles;
+    if (split < 1) split = 1;
+    const int tiles_per = (jtiles + split - 1) / split;
+    s.jchunk = tiles_per * block;
+    s.jsplit = (n + s.jchunk - 1) / s.jchunk;
+    return s;
+}
+
+int check_cfg(const nbh_config* cfg)
+{
+    if (!cfg) return fail(NBH_ERR_ARG, "config is NULL");
+    if (cfg->precision != NBH_FP64 && cfg->precision != NBH_MIXED)
+        return fail(NBH_ERR_ARG, "precision must be NBH_FP64 or NBH_MIXED, got %d", cfg->precision);
+    if (!(cfg->eps2 >= 0.0) || !std::isfinite(cfg->eps2))
+        return fail(NBH_ERR_ARG, "eps2 must be finite and >= 0, got %g", cfg->eps2);
+    if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256)
+        return fail(NBH_ERR_ARG, "block must be 0, 64, 128 or 256, got %d", cfg->block);
+    if (cfg->jsplit < 0) return fail(NBH_ERR_ARG, "jsplit must be >= 0, got %d", cfg->jsplit);
+    return NBH_OK;
+}
+
+void drop_graph(nbh_ctx* c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_steps = 0;
+    c->g_n = -1;
+}
+
+nbh::State state_of(double* base, int n)
+{
+    nbh::State s;
+    s.x = base;
+    s.y = base + (size_t)n;
+    s.z = base + 2 * (size_t)n;
+    s.m = base + 3 * (size_t)n;
+    s.vx = base + 4 * (size_t)n;
+    s.vy = base + 5 * (size_t)n;
+    s.vz = base + 6 * (size_t)n;
+    return s;
+}
+
+nbh::Forces forces_of(double* base, int n)
+{
+    nbh::Forces f;
+    for (int c = 0; c < 6; ++c) f.c[c] = base + (size_t)c * n;
+    return f;
+}
+
+int get_events(nbh_ctx* c, EventPair* ep)
+{
+    if (!c->ev_free.empty()) {
+        *ep = c->ev_free.back();
+        c->ev_free.pop_back();
+        return NBH_OK;
+    }
+    NBH_HIP(hipEventCreate(&ep->a));
+    NBH_HIP(hipEventCreate(&ep->b));
+    return NBH_OK;
+}
+
+// Adds the elapsed time of every recorded pair (stream must be idle).
+int settle_events(nbh_ctx* c)
+{
+    for (auto& ep : c->ev_pending) {
+        float ms = 0.f;
+        NBH_HIP(hipEventElapsedTime(&ms, ep.a, ep.b));
+        c->force_ms += ms;
+        c->force_launches += 1;
+        c->ev_free.push_back(ep);
+    }
+    c->ev_pending.clear();
+    return NBH_OK;
+}
+
+// One force evaluation on device arrays, enqueued on st.
+int enqueue_force(nbh_ctx* c, const nbh::State& s, int n, const nbh::Forces& out,
+                  hipStream_t st, bool allow_events)
+{
+    const nbh::Shape sh = pick_shape(c, n);
+    double* part = nullptr;
+    if (sh.jsplit > 1) {
+        NBH_HIP(c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n));
+        part = c->partial.d();
+    }
+    EventPair ep;
+    const bool timed = c->profiling && allow_events;
+    if (timed) {
+        int rc = get_events(c, &ep);
+        if (rc) return rc;
+    }
+    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, out, part, st,
+                                     timed ? ep.a : nullptr, timed ? ep.b : nullptr);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "force launch failed: %s", hipGetErrorString(e));
+    if (timed) c->ev_pending.push_back(ep);
+    return NBH_OK;
+}
+
+bool bad_n(int n) { return n < 1; }
+
+}  // namespace
+
+extern "C" {
+
+const char* nbh_last_error(void) { return g_err.c_str(); }
+
+int nbh_version(void) { return 100; }
+
+int nbh_device_info(int device, int* compute_units, int* wave_size, char* name, int name_len)
+{
+    int count = 0;
+    NBH_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(NBH_ERR_ARG, "device %d out of range (%d devices)", device, count);
+    hipDeviceProp_t p;
+    NBH_HIP(hipGetDeviceProperties(&p, device));
+    if (compute_units) *compute_units = p.multiProcessorCount;
+    if (wave_size) *wave_size = p.warpSize;
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s", p.gcnArchName);
+    }
+    return NBH_OK;
+}
+
+int nbh_create(const nbh_config* cfg, nbh_ctx** out)
+{
+    if (!out) return fail(NBH_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return fail(NBH_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (cfg->device < 0 || cfg->device >= count)
+        return fail(NBH_ERR_ARG, "device %d out of range (%d devices)", cfg->device, count);
+    NBH_HIP(hipSetDevice(cfg->device));
+    hipDeviceProp_t p;
+    NBH_HIP(hipGetDeviceProperties(&p, cfg->device));
+    auto* c = new (std::nothrow) nbh_ctx();
+    if (!c) return fail(NBH_ERR_ALLOC, "context allocation failed");
+    c->cfg = *cfg;
+    c->cus = p.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(NBH_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    e = c->scal.reserve(sizeof(double) * 8);
+    if (e == hipSuccess) e = c->flag.reserve(sizeof(int) * 2);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(NBH_ERR_ALLOC, "device allocation failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return NBH_OK;
+}
+
+int nbh_destroy(nbh_ctx* c)
+{
+    if (!c) return NBH_OK;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (auto& ep : c->ev_pending) c->ev_free.push_back(ep);
+    for (auto& ep : c->ev_free) {
+        (void)hipEventDestroy(ep.a);
+        (void)hipEventDestroy(ep.b);
+    }
+    for (DevBuf* b : {&c->in, &c->out, &c->partial, &c->flag, &c->scal, &c->state, &c->f0,
+                      &c->f1, &c->ework})
+        b->release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NBH_OK;
+}
+
+int nbh_configure(nbh_ctx* c, const nbh_config* cfg)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    if (cfg->device != c->cfg.device)
+        return fail(NBH_ERR_ARG, "cannot move a context to another device");
+    c->cfg = *cfg;
+    drop_graph(c);
+    return NBH_OK;
+}
+
+int nbh_launch_shape(nbh_ctx* c, int n, int* block, int* jsplit, int* workgroups)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    const nbh::Shape s = pick_shape(c, n);
+    if (block) *block = s.block;
+    if (jsplit) *jsplit = s.jsplit;
+    if (workgroups) *workgroups = ((n + s.block - 1) / s.block) * s.jsplit;
+    return NBH_OK;
+}
+
+int nbh_force(nbh_ctx* c, int n, const double* x, const double* y, const double* z,
+              const double* m, const double* vx, const double* vy, const double* vz,
+              double* ax, double* ay, double* az, double* jx, double* jy, double* jz,
+              double* kernel_ms)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    const double* ins[7] = {x, y, z, m, vx, vy, vz};
+    double* outs[6] = {ax, ay, az, jx, jy, jz};
+    for (auto p : ins)
+        if (!p) return fail(NBH_ERR_ARG, "input array is NULL");
+    for (auto p : outs)
+        if (!p) return fail(NBH_ERR_ARG, "output array is NULL");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const size_t nb = sizeof(double) * (size_t)n;
+    if (c->in.reserve(7 * nb) != hipSuccess || c->out.reserve(6 * nb) != hipSuccess)
+        return fail(NBH_ERR_ALLOC, "device allocation of %zu bytes failed", 13 * nb);
+    for (int k = 0; k < 7; ++k)
+        NBH_HIP(hipMemcpyAsync(c->in.d() + (size_t)k * n, ins[k], nb, hipMemcpyHostToDevice, c->stream));
+    const nbh::State s = state_of(c->in.d(), n);
+    const nbh::Forces f = forces_of(c->out.d(), n);
+    // the host call times its own kernels regardless of the profiling switch
+    EventPair ep;
+    int rc = get_events(c, &ep);
+    if (rc) return rc;
+    const nbh::Shape sh = pick_shape(c, n);
+    double* part = nullptr;
+    if (sh.jsplit > 1) {
+        if (c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n) != hipSuccess)
+            return fail(NBH_ERR_ALLOC, "device allocation of split partials failed");
+        part = c->partial.d();
+    }
+    NBH_HIP(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
+    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, f, part, c->stream,
+                                     ep.a, ep.b);
+    if (e != hipSuccess) {
+        c->ev_free.push_back(ep);
+        return fail(NBH_ERR_HIP, "force launch failed: %s", hipGetErrorString(e));
+    }
+    e = nbh::launch_check_finite(n, f, static_cast<int*>(c->flag.p), c->stream);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "finite check failed: %s", hipGetErrorString(e));
+    for (int k = 0; k < 6; ++k)
+        NBH_HIP(hipMemcpyAsync(outs[k], c->out.d() + (size_t)k * n, nb, hipMemcpyDeviceToHost, c->stream));
+    int flag = 0;
+    NBH_HIP(hipMemcpyAsync(&flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    NBH_HIP(hipEventElapsedTime(&ms, ep.a, ep.b));
+    c->ev_free.push_back(ep);
+    if (c->profiling) {
+        c->force_ms += ms;
+        c->force_launches += 1;
+    }
+    if (kernel_ms) *kernel_ms = ms;
+    if (flag) return fail(NBH_ERR_NONFINITE, "force result holds inf/NaN (n=%d, eps2=%g)", n, c->cfg.eps2);
+    return NBH_OK;
+}
+
+int nbh_force_device(nbh_ctx* c, int n, const double* x, const double* y, const double* z,
+                     const double* m, const double* vx, const double* vy, const double* vz,
+                     double* ax, double* ay, double* az, double* jx, double* jy, double* jz,
+                     void* stream)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    const double* ins[7] = {x, y, z, m, vx, vy, vz};
+    double* outs[6] = {ax, ay, az, jx, jy, jz};
+    for (auto p : ins)
+        if (!p) return fail(NBH_ERR_ARG, "input array is NULL");
+    for (auto p : outs)
+        if (!p) return fail(NBH_ERR_ARG, "output array is NULL");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    nbh::State s{x, y, z, m, vx, vy, vz};
+    nbh::Forces f;
+    for (int k = 0; k < 6; ++k) f.c[k] = outs[k];
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    // events are only timed on the context's own stream, which nbh_timing drains
+    return enqueue_force(c, s, n, f, st, st == c->stream);
+}
+
+int nbh_upload(nbh_ctx* c, int n, const double* x, const double* y, const double* z,
+               const double* m, const double* vx, const double* vy, const double* vz)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    const double* ins[7] = {x, y, z, m, vx, vy, vz};
+    for (auto p : ins)
+        if (!p) return fail(NBH_ERR_ARG, "input array is NULL");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const size_t nb = sizeof(double) * (size_t)n;
+    drop_graph(c);
+    if (c->state.reserve(7 * nb) != hipSuccess || c->f0.reserve(6 * nb) != hipSuccess ||
+        c->f1.reserve(6 * nb) != hipSuccess)
+        return fail(NBH_ERR_ALLOC, "device allocation for n=%d failed", n);
+    c->n = n;
+    for (int k = 0; k < 7; ++k)
+        NBH_HIP(hipMemcpyAsync(c->state.d() + (size_t)k * n, ins[k], nb, hipMemcpyHostToDevice, c->stream));
+    int rc = enqueue_force(c, state_of(c->state.d(), n), n, forces_of(c->f0.d(), n), c->stream, false);
+    if (rc) return rc;
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    c->dt_set = false;
+    return NBH_OK;
+}
+
+int nbh_initial_dt(nbh_ctx* c, double eta0, double* dt)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (!dt) return fail(NBH_ERR_ARG, "dt is NULL");
+    if (c->n == 0) return fail(NBH_ERR_STATE, "no resident state: call nbh_upload first");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    auto* slot = reinterpret_cast<unsigned long long*>(c->scal.d() + 4);
+    hipError_t e = nbh::launch_fill_inf(slot, c->stream);
+    if (e == hipSuccess) e = nbh::launch_initial_dt(c->n, forces_of(c->f0.d(), c->n), eta0, slot, c->stream);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "initial dt failed: %s", hipGetErrorString(e));
+    NBH_HIP(hipMemcpyAsync(dt, slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    return NBH_OK;
+}
+
+int nbh_set_dt(nbh_ctx* c, double dt, int adaptive, double eta)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(NBH_ERR_ARG, "dt must be finite and > 0, got %g", dt);
+    if (adaptive && !(eta > 0.0)) return fail(NBH_ERR_ARG, "eta must be > 0, got %g", eta);
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    double v[2];
+    v[0] = dt;
+    const double inf = INFINITY;
+    std::memcpy(&v[1], &inf, sizeof inf);
+    NBH_HIP(hipMemcpyAsync(c->scal.d(), v, sizeof v, hipMemcpyHostToDevice, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    if ((adaptive != 0) != c->adaptive || eta != c->eta) drop_graph(c);
+    c->adaptive = adaptive != 0;
+    c->eta = eta;
+    c->dt_set = true;
+    return NBH_OK;
+}
+
+int nbh_get_dt(nbh_ctx* c, double* dt)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (!dt) return fail(NBH_ERR_ARG, "dt is NULL");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    NBH_HIP(hipMemcpyAsync(dt, c->scal.d(), sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    return NBH_OK;
+}
+
+namespace {
+
+// Enqueues one Hermite step of the resident state on c->stream.
+int enqueue_step(nbh_ctx* c, bool allow_events)
+{
+    const int n = c->n;
+    double* b = c->state.d();
+    const nbh::Forces f0 = forces_of(c->f0.d(), n);
+    const nbh::Forces f1 = forces_of(c->f1.d(), n);
+    double* dt = c->scal.d();
+    auto* dt_next = reinterpret_cast<unsigned long long*>(c->scal.d() + 1);
+    double *x = b, *y = b + n, *z = b + 2 * (size_t)n;
+    double *vx = b + 4 * (size_t)n, *vy = b + 5 * (size_t)n, *vz = b + 6 * (size_t)n;
+    hipError_t e = nbh::launch_predict(n, dt, x, y, z, vx, vy, vz, f0, c->stream);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "predict launch failed: %s", hipGetErrorString(e));
+    int rc = enqueue_force(c, state_of(b, n), n, f1, c->stream, allow_events);
+    if (rc) return rc;
+    e = nbh::launch_correct(n, dt, x, y, z, vx, vy, vz, f0, f1, c->adaptive ? dt_next : nullptr,
+                            c->eta, c->stream);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "correct launch failed: %s", hipGetErrorString(e));
+    if (c->adaptive) {
+        e = nbh::launch_dt_advance(dt, dt_next, c->stream);
+        if (e != hipSuccess) return fail(NBH_ERR_HIP, "dt advance failed: %s", hipGetErrorString(e));
+    }
+    return NBH_OK;
+}
+
+int build_graph(nbh_ctx* c, int steps)
+{
+    drop_graph(c);
+    NBH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = NBH_OK;
+    for (int k = 0; k < steps && rc == NBH_OK; ++k) rc = enqueue_step(c, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(NBH_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    c->g_steps = steps;
+    c->g_n = c->n;
+    c->g_adaptive = c->adaptive;
+    return NBH_OK;
+}
+
+}  // namespace
+
+int nbh_step(nbh_ctx* c, int nsteps, int use_graph)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (nsteps < 0) return fail(NBH_ERR_ARG, "nsteps must be >= 0, got %d", nsteps);
+    if (c->n == 0) return fail(NBH_ERR_STATE, "no resident state: call nbh_upload first");
+    if (!c->dt_set) return fail(NBH_ERR_STATE, "no time step: call nbh_set_dt first");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const auto t0 = std::chrono::steady_clock::now();
+    int done = 0;
+    if (use_graph && nsteps > 0) {
+        const int chunk = nsteps < 32 ? nsteps : 32;
+        if (!c->gexec || c->g_steps != chunk || c->g_n != c->n || c->g_adaptive != c->adaptive) {
+            int rc = build_graph(c, chunk);
+            if (rc) return rc;
+        }
+        while (nsteps - done >= c->g_steps) {
+            NBH_HIP(hipGraphLaunch(c->gexec, c->stream));
+            done += c->g_steps;
+        }
+    }
+    for (; done < nsteps; ++done) {
+        int rc = enqueue_step(c, true);
+        if (rc) return rc;
+    }
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    c->step_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    c->steps += nsteps;
+    return settle_events(c);
+}
+
+int nbh_download(nbh_ctx* c, double* x, double* y, double* z, double* m, double* vx, double* vy,
+                 double* vz, double* ax, double* ay, double* az, double* jx, double* jy, double* jz)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (c->n == 0) return fail(NBH_ERR_STATE, "no resident state: call nbh_upload first");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const int n = c->n;
+    const size_t nb = sizeof(double) * (size_t)n;
+    double* so[7] = {x, y, z, m, vx, vy, vz};
+    double* fo[6] = {ax, ay, az, jx, jy, jz};
+    for (int k = 0; k < 7; ++k)
+        if (so[k])
+            NBH_HIP(hipMemcpyAsync(so[k], c->state.d() + (size_t)k * n, nb, hipMemcpyDeviceToHost, c->stream));
+    for (int k = 0; k < 6; ++k)
+        if (fo[k])
+            NBH_HIP(hipMemcpyAsync(fo[k], c->f0.d() + (size_t)k * n, nb, hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    return NBH_OK;
+}
+
+namespace {
+
+int energy_of(nbh_ctx* c, const nbh::State& s, int n, int compat, double* kin, double* pot)
+{
+    const int nb = nbh::energy_blocks(n);
+    if (c->ework.reserve(sizeof(double) * 2 * (size_t)nb) != hipSuccess)
+        return fail(NBH_ERR_ALLOC, "device allocation for energy failed");
+    double* out = c->scal.d() + 2;
+    const double eps2 = compat ? 1.0e-7 : c->cfg.eps2;
+    hipError_t e = nbh::launch_energy(s, n, eps2, compat, c->ework.d(), out, c->stream);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "energy launch failed: %s", hipGetErrorString(e));
+    double v[2];
+    NBH_HIP(hipMemcpyAsync(v, out, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    if (kin) *kin = v[0];
+    if (pot) *pot = v[1];
+    return NBH_OK;
+}
+
+}  // namespace
+
+int nbh_energy(nbh_ctx* c, int compat, double* kin, double* pot)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (c->n == 0) return fail(NBH_ERR_STATE, "no resident state: call nbh_upload first");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    return energy_of(c, state_of(c->state.d(), c->n), c->n, compat, kin, pot);
+}
+
+int nbh_energy_host(nbh_ctx* c, int n, const double* x, const double* y, const double* z,
+                    const double* m, const double* vx, const double* vy, const double* vz,
+                    int compat, double* kin, double* pot)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    const double* ins[7] = {x, y, z, m, vx, vy, vz};
+    for (auto p : ins)
+        if (!p) return fail(NBH_ERR_ARG, "input array is NULL");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const size_t nb = sizeof(double) * (size_t)n;
+    if (c->in.reserve(7 * nb) != hipSuccess) return fail(NBH_ERR_ALLOC, "device allocation failed");
+    for (int k = 0; k < 7; ++k)
+        NBH_HIP(hipMemcpyAsync(c->in.d() + (size_t)k * n, ins[k], nb, hipMemcpyHostToDevice, c->stream));
+    return energy_of(c, state_of(c->in.d(), n), n, compat, kin, pot);
+}
+
+int nbh_set_profiling(nbh_ctx* c, int on)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    c->profiling = on != 0;
+    return NBH_OK;
+}
+
+int nbh_timing(nbh_ctx* c, double* force_ms, int64_t* force_launches, double* step_ms, int64_t* steps)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    int rc = settle_events(c);
+    if (rc) return rc;
+    if (force_ms) *force_ms = c->force_ms;
+    if (force_launches) *force_launches = c->force_launches;
+    if (step_ms) *step_ms = c->step_ms;
+    if (steps) *steps = c->steps;
+    return NBH_OK;
+}
+
+int nbh_reset_timing(nbh_ctx* c)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    int rc = nbh_timing(c, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    c->force_ms = 0.0;
+    c->force_launches = 0;
+    c->step_ms = 0.0;
+    c->steps = 0;
+    return NBH_OK;
+}
+
+}  // extern "C"

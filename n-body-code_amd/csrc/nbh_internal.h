@@ -1,0 +1,78 @@
+// nbh_internal.h -- shared between the kernels (nbh_kernels.hip) and the
+// C-ABI layer (nbh_api.hip).  Not installed; the public surface is
+// include/nbody_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace nbh {
+
+// Body state in HBM: seven SoA double[n] arrays (x, y, z, m, vx, vy, vz).
+struct State {
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* m;
+    const double* vx;
+    const double* vy;
+    const double* vz;
+};
+
+// Six SoA output arrays (ax, ay, az, jx, jy, jz).
+struct Forces {
+    double* c[6];
+};
+
+// One force evaluation's launch shape.
+struct Shape {
+    int block;   // threads per workgroup = i-bodies per workgroup = j-tile length
+    int jsplit;  // workgroups along j; partial sums reduced in a second pass
+    int jchunk;  // j-bodies per split (multiple of block)
+};
+
+// Pair-kernel launch: writes forces for all n targets from all n sources.
+// When shape.jsplit > 1, `partial` must hold 6*jsplit*n doubles and the
+// split-reduction kernel is launched after the pair kernel.  ev0/ev1 (may be
+// null) are recorded around the pair kernel only.
+hipError_t launch_force(int precision, const State& s, int n, double eps2,
+                        const Shape& shape, const Forces& out,
+                        double* partial, hipStream_t st, hipEvent_t ev0,
+                        hipEvent_t ev1);
+
+// Hermite predictor in place (x, v) from forces f0 with dt read from *dt.
+hipError_t launch_predict(int n, const double* dt, double* x, double* y,
+                          double* z, double* vx, double* vy, double* vz,
+                          const Forces& f0, hipStream_t st);
+
+// Snap/crackle + corrector in place; then f0 <- f1.  With dt_next non-null
+// the Aarseth candidate (eta) is min-reduced into *dt_next (as u64 bits).
+hipError_t launch_correct(int n, const double* dt, double* x, double* y,
+                          double* z, double* vx, double* vy, double* vz,
+                          const Forces& f0, const Forces& f1,
+                          unsigned long long* dt_next, double eta,
+                          hipStream_t st);
+
+// *dt = bits(*dt_next); *dt_next = +inf bits.
+hipError_t launch_dt_advance(double* dt, unsigned long long* dt_next,
+                             hipStream_t st);
+
+// *out = bits(min_i eta0*|a_i|/|j_i|), *out pre-set by the caller.
+hipError_t launch_initial_dt(int n, const Forces& f, double eta0,
+                             unsigned long long* out, hipStream_t st);
+
+// Fills *p with +inf bits (a kernel, so it is safe inside graph capture).
+hipError_t launch_fill_inf(unsigned long long* p, hipStream_t st);
+
+// Energy: out[0] = kinetic, out[1] = potential.  `work` holds
+// 2*energy_blocks(n) doubles.
+int energy_blocks(int n);
+hipError_t launch_energy(const State& s, int n, double eps2, int compat,
+                         double* work, double* out, hipStream_t st);
+
+// Sets *flag to 1 if any of the 6n force values is not finite.
+hipError_t launch_check_finite(int n, const Forces& f, int* flag,
+                               hipStream_t st);
+
+}  // namespace nbh

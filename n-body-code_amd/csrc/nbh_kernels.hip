@@ -1,0 +1,633 @@
+// nbh_kernels.hip -- gfx950 kernels for the Hermite-4 force path.
+//
+// Pair kernels (the hot path, replacing the reference's force loops
+// src/kernel.cpp:1101-1224 and the Tensix nb_compute pipeline
+// kernels/nb_compute.cpp:180-582):
+//   * one target (i) body per lane, BLOCK lanes per workgroup;
+//   * source (j) bodies streamed from HBM in tiles of BLOCK bodies: each lane
+//     loads one body with coalesced SoA loads and stores it into an
+//     LDS tile (array-of-structs, 16-byte aligned); two tiles alternate so the
+//     global loads of tile t+1 overlap the arithmetic on tile t;
+//   * the inner loop reads tile entries at a wave-uniform address, so every
+//     LDS read is a broadcast to the 64 lanes;
+//   * grid.y splits the j-range when n is too small to fill the CUs; the
+//     partial sums are added by a second kernel in split order, so results
+//     are deterministic run to run;
+//   * no MFMA: the softened r^-3 interaction is rsqrt-bound VALU work.
+//
+// Per pair (directed i <- j), both precisions:
+//   d = x_j - x_i, u = v_j - v_i, s = |d|^2 + eps2, rv = d.u,
+//   r = s^-1/2, a_i += m_j r^3 d, jerk_i += m_j r^3 (u - 3 rv r^2 d)
+// which is the reference's per-pair formula (src/kernel.cpp:1188-1205)
+// summed over all j != i instead of over the j < i triangle.
+#include "nbh_internal.h"
+
+#include <cmath>
+
+namespace nbh {
+namespace {
+
+constexpr int kOBlock = 256;  // threads per workgroup of the O(n) kernels
+
+// FP64 source body in LDS: 64 bytes, read as three 16-byte and one 8-byte
+// broadcast (pad is never read).
+struct __align__(16) JB64 {
+    double x, y, z, m, vx, vy, vz, pad;
+};
+
+// Mixed-precision source body in LDS: FP32 velocity and mass first, FP64
+// position after, 48 bytes -> two 16-byte and one 8-byte broadcast.
+struct __align__(16) JBMX {
+    float vx, vy, vz, m;
+    double x, y, z, pad;
+};
+
+// s^-1/2 in FP64: v_rsq_f64 seed refined by one third-order step,
+// y' = y + y e (1/2 + 3/8 e) with e = 1 - s y^2.  The seed's relative error
+// d becomes O(d^3) plus the rounding of e, i.e. about one ulp.
+__device__ __forceinline__ double rsqrt_f64(double s)
+{
+    const double y = __builtin_amdgcn_rsq(s);
+    const double e = fma(-s, y * y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+struct PairArgs {
+    State s;
+    int n;
+    int jchunk;
+    double eps2;
+    Forces out;
+    size_t split_stride;  // element offset between j-split partial planes
+};
+
+__device__ __forceinline__ JB64 load_jb64(const State& s, int j)
+{
+    JB64 b;
+    b.x = s.x[j];
+    b.y = s.y[j];
+    b.z = s.z[j];
+    b.m = s.m[j];
+    b.vx = s.vx[j];
+    b.vy = s.vy[j];
+    b.vz = s.vz[j];
+    b.pad = 0.0;
+    return b;
+}
+
+__device__ __forceinline__ JBMX load_jbmx(const State& s, int j)
+{
+    JBMX b;
+    b.vx = (float)s.vx[j];
+    b.vy = (float)s.vy[j];
+    b.vz = (float)s.vz[j];
+    b.m = (float)s.m[j];
+    b.x = s.x[j];
+    b.y = s.y[j];
+    b.z = s.z[j];
+    b.pad = 0.0;
+    return b;
+}
+
+// ---------------------------------------------------------------- FP64 --
+template <bool MASK>
+__device__ __forceinline__ void pair_f64(const JB64& b, int j, int i,
+                                         double xi, double yi, double zi,
+                                         double vxi, double vyi, double vzi,
+                                         double eps2, double (&acc)[6])
+{
+    const double dx = b.x - xi;
+    const double dy = b.y - yi;
+    const double dz = b.z - zi;
+    const double ux = b.vx - vxi;
+    const double uy = b.vy - vyi;
+    const double uz = b.vz - vzi;
+    double s = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+    const double vr = fma(dx, ux, fma(dy, uy, dz * uz));
+    double mj = b.m;
+    if (MASK) {  // eps2 == 0: drop the self pair instead of dividing by 0
+        const bool self = (j == i);
+        s = self ? 1.0 : s;
+        mj = self ? 0.0 : mj;
+    }
+    const double r = rsqrt_f64(s);
+    const double r2 = r * r;
+    const double mr3 = mj * r * r2;
+    const double al = -3.0 * vr * r2;
+    acc[0] = fma(mr3, dx, acc[0]);
+    acc[1] = fma(mr3, dy, acc[1]);
+    acc[2] = fma(mr3, dz, acc[2]);
+    acc[3] = fma(mr3, fma(al, dx, ux), acc[3]);
+    acc[4] = fma(mr3, fma(al, dy, uy), acc[4]);
+    acc[5] = fma(mr3, fma(al, dz, uz), acc[5]);
+}
+
+template <int BLOCK, bool MASK>
+__global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
+{
+    __shared__ JB64 tile[2][BLOCK];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * BLOCK + t;
+    const int ic = i < a.n ? i : a.n - 1;
+    const double xi = a.s.x[ic], yi = a.s.y[ic], zi = a.s.z[ic];
+    const double vxi = a.s.vx[ic], vyi = a.s.vy[ic], vzi = a.s.vz[ic];
+    const double eps2 = a.eps2;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+
+    const int jbeg = blockIdx.y * a.jchunk;
+    const int jend = min(a.n, jbeg + a.jchunk);
+    JB64 st{};
+    if (jbeg + t < jend) st = load_jb64(a.s, jbeg + t);
+    tile[0][t] = st;
+    __syncthreads();
+    int buf = 0;
+    for (int jt = jbeg; jt < jend; jt += BLOCK) {
+        const int cnt = min(BLOCK, jend - jt);
+        const bool more = jt + BLOCK < jend;
+        if (more && jt + BLOCK + t < jend) st = load_jb64(a.s, jt + BLOCK + t);
+        const JB64* tb = tile[buf];
+        if (cnt == BLOCK) {
+#pragma unroll 4
+            for (int k = 0; k < BLOCK; ++k)
+                pair_f64<MASK>(tb[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+        } else {
+            for (int k = 0; k < cnt; ++k)
+                pair_f64<MASK>(tb[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+        }
+        if (more) tile[buf ^ 1][t] = st;
+        __syncthreads();
+        buf ^= 1;
+    }
+    if (i < a.n) {
+        const size_t o = (size_t)blockIdx.y * a.split_stride + i;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.out.c[c][o] = acc[c];
+    }
+}
+
+// --------------------------------------------------------------- mixed --
+// Position differences are formed in FP64 and rounded; everything after is
+// FP32.  Terms are summed in FP32 over runs of kFold sources, and each run
+// total is folded into an FP32 (hi, lo) pair with Knuth's two-sum, so the
+// rounding error of the long sum is carried in lo.
+constexpr int kFold = 64;
+
+__device__ __forceinline__ void two_sum_fold(float& hi, float& lo, float v)
+{
+    const float s = hi + v;
+    const float bv = s - hi;
+    const float err = (hi - (s - bv)) + (v - bv);
+    hi = s;
+    lo += err;
+}
+
+template <bool MASK>
+__device__ __forceinline__ void pair_mx(const JBMX& b, int j, int i,
+                                        double xi, double yi, double zi,
+                                        float vxi, float vyi, float vzi,
+                                        float eps2, float (&acc)[6])
+{
+    const float dx = (float)(b.x - xi);
+    const float dy = (float)(b.y - yi);
+    const float dz = (float)(b.z - zi);
+    const float ux = b.vx - vxi;
+    const float uy = b.vy - vyi;
+    const float uz = b.vz - vzi;
+    float s = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+    const float vr = fmaf(dx, ux, fmaf(dy, uy, dz * uz));
+    float mj = b.m;
+    if (MASK) {
+        const bool self = (j == i);
+        s = self ? 1.0f : s;
+        mj = self ? 0.0f : mj;
+    }
+    const float r = __builtin_amdgcn_rsqf(s);
+    const float r2 = r * r;
+    const float mr3 = mj * r * r2;
+    const float al = -3.0f * vr * r2;
+    acc[0] = fmaf(mr3, dx, acc[0]);
+    acc[1] = fmaf(mr3, dy, acc[1]);
+    acc[2] = fmaf(mr3, dz, acc[2]);
+    acc[3] = fmaf(mr3, fmaf(al, dx, ux), acc[3]);
+    acc[4] = fmaf(mr3, fmaf(al, dy, uy), acc[4]);
+    acc[5] = fmaf(mr3, fmaf(al, dz, uz), acc[5]);
+}
+
+template <int BLOCK, bool MASK>
+__global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a)
+{
+    static_assert(BLOCK % kFold == 0, "tile must hold whole fold runs");
+    __shared__ JBMX tile[2][BLOCK];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * BLOCK + t;
+    const int ic = i < a.n ? i : a.n - 1;
+    const double xi = a.s.x[ic], yi = a.s.y[ic], zi = a.s.z[ic];
+    const float vxi = (float)a.s.vx[ic], vyi = (float)a.s.vy[ic], vzi = (float)a.s.vz[ic];
+    const float eps2 = (float)a.eps2;
+    float hi[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float lo[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+
+    const int jbeg = blockIdx.y * a.jchunk;
+    const int jend = min(a.n, jbeg + a.jchunk);
+    JBMX st{};
+    if (jbeg + t < jend) st = load_jbmx(a.s, jbeg + t);
+    tile[0][t] = st;
+    __syncthreads();
+    int buf = 0;
+    for (int jt = jbeg; jt < jend; jt += BLOCK) {
+        const int cnt = min(BLOCK, jend - jt);
+        const bool more = jt + BLOCK < jend;
+        if (more && jt + BLOCK + t < jend) st = load_jbmx(a.s, jt + BLOCK + t);
+        const JBMX* tb = tile[buf];
+        for (int k0 = 0; k0 < cnt; k0 += kFold) {
+            float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (k0 + kFold <= cnt) {
+#pragma unroll 4
+                for (int k = 0; k < kFold; ++k)
+                    pair_mx<MASK>(tb[k0 + k], jt + k0 + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+            } else {
+                for (int k = k0; k < cnt; ++k)
+                    pair_mx<MASK>(tb[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) two_sum_fold(hi[c], lo[c], acc[c]);
+        }
+        if (more) tile[buf ^ 1][t] = st;
+        __syncthreads();
+        buf ^= 1;
+    }
+    if (i < a.n) {
+        const size_t o = (size_t)blockIdx.y * a.split_stride + i;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.out.c[c][o] = (double)hi[c] + (double)lo[c];
+    }
+}
+
+// Second pass of a j-split: out[c][i] = sum over splits in order 0..S-1.
+__global__ __launch_bounds__(kOBlock) void k_reduce_splits(const double* __restrict__ part,
+                                                           int n, int splits, Forces out)
+{
+    const int i = blockIdx.x * kOBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n) return;
+    const double* p = part + (size_t)c * splits * n + i;
+    double v = 0.0;
+    for (int s = 0; s < splits; ++s) v += p[(size_t)s * n];
+    out.c[c][i] = v;
+}
+
+// ------------------------------------------------------- integrator -----
+// Predictor, src/nbody.cpp:152-163, same operations in the same order and
+// no contraction, so results are bitwise those of the reference's x86-64
+// build (which has no FMA).
+__global__ __launch_bounds__(kOBlock) void k_predict(int n, const double* __restrict__ dtp,
+                                                     double* __restrict__ x, double* __restrict__ y,
+                                                     double* __restrict__ z, double* __restrict__ vx,
+                                                     double* __restrict__ vy, double* __restrict__ vz,
+                                                     Forces f)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kOBlock + threadIdx.x;
+    if (i >= n) return;
+    const double dt = *dtp;
+    const double ax = f.c[0][i], ay = f.c[1][i], az = f.c[2][i];
+    const double jx = f.c[3][i], jy = f.c[4][i], jz = f.c[5][i];
+    const double ux = vx[i], uy = vy[i], uz = vz[i];
+    x[i] = x[i] + (ux * dt + 0.5 * ax * dt * dt + 1.0 / 6.0 * jx * dt * dt * dt);
+    y[i] = y[i] + (uy * dt + 0.5 * ay * dt * dt + 1.0 / 6.0 * jy * dt * dt * dt);
+    z[i] = z[i] + (uz * dt + 0.5 * az * dt * dt + 1.0 / 6.0 * jz * dt * dt * dt);
+    vx[i] = ux + (ax * dt + 0.5 * jx * dt * dt);
+    vy[i] = uy + (ay * dt + 0.5 * jy * dt * dt);
+    vz[i] = uz + (az * dt + 0.5 * jz * dt * dt);
+}
+
+__device__ __forceinline__ void block_min_u64(unsigned long long v, unsigned long long* out)
+{
+    __shared__ unsigned long long red[kOBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_down(v, off, 64);
+        v = o < v ? o : v;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = red[0];
+        for (int k = 1; k < kOBlock / 64; ++k) m = red[k] < m ? red[k] : m;
+        atomicMin(out, m);
+    }
+}
+
+__device__ __forceinline__ unsigned long long dt_bits(double d)
+{
+    // Non-negative doubles order like their bit patterns; NaN sorts last.
+    return (d >= 0.0) ? (unsigned long long)__double_as_longlong(d) : ~0ull;
+}
+
+// Snap/crackle and corrector, src/nbody.cpp:165-196 (no contraction, same
+// order), then the pointer swap of :263-288 done as a copy f0 <- f1 so the
+// captured step sequence keeps fixed addresses.  With dt_next set, the
+// Aarseth criterion of NBody::updatedt (src/nbody.cpp:198-222), which uses
+// the start-of-step a0, j0 and the new a2, a3, is min-reduced.
+__global__ __launch_bounds__(kOBlock) void k_correct(int n, const double* __restrict__ dtp,
+                                                     double* __restrict__ x, double* __restrict__ y,
+                                                     double* __restrict__ z, double* __restrict__ vx,
+                                                     double* __restrict__ vy, double* __restrict__ vz,
+                                                     Forces f0, Forces f1,
+                                                     unsigned long long* dt_next, double eta)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kOBlock + threadIdx.x;
+    unsigned long long cand = ~0ull;
+    if (i < n) {
+        const double dt = *dtp;
+        const double invdt2 = 1.0 / (dt * dt);
+        const double invdt3 = 1.0 / (dt * dt * dt);
+        double a2[3], a3[3], a0[3], j0[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a0[c] = f0.c[c][i];
+            j0[c] = f0.c[3 + c][i];
+            const double a1 = f1.c[c][i];
+            const double j1 = f1.c[3 + c][i];
+            a2[c] = 2.0 * invdt2 * (-3.0 * (a0[c] - a1) - (2.0 * j0[c] + j1) * dt);
+            a3[c] = 6.0 * invdt3 * (2.0 * (a0[c] - a1) + (j0[c] + j1) * dt);
+        }
+        const double dt3 = dt * dt * dt;
+        const double dt4 = dt3 * dt;
+        const double dt5 = dt4 * dt;
+        x[i] = x[i] + (1.0 / 24.0 * a2[0] * dt4 + 1.0 / 120.0 * a3[0] * dt5);
+        y[i] = y[i] + (1.0 / 24.0 * a2[1] * dt4 + 1.0 / 120.0 * a3[1] * dt5);
+        z[i] = z[i] + (1.0 / 24.0 * a2[2] * dt4 + 1.0 / 120.0 * a3[2] * dt5);
+        vx[i] = vx[i] + (1.0 / 6.0 * a2[0] * dt3 + 1.0 / 24.0 * a3[0] * dt4);
+        vy[i] = vy[i] + (1.0 / 6.0 * a2[1] * dt3 + 1.0 / 24.0 * a3[1] * dt4);
+        vz[i] = vz[i] + (1.0 / 6.0 * a2[2] * dt3 + 1.0 / 24.0 * a3[2] * dt4);
+        if (dt_next) {
+            const double na2 = a0[0] * a0[0] + a0[1] * a0[1] + a0[2] * a0[2];
+            const double na = sqrt(na2);
+            const double nj2 = j0[0] * j0[0] + j0[1] * j0[1] + j0[2] * j0[2];
+            const double nj = sqrt(nj2);
+            const double ns2 = a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2];
+            const double ns = sqrt(ns2);
+            const double nc2 = a3[0] * a3[0] + a3[1] * a3[1] + a3[2] * a3[2];
+            const double nc = sqrt(nc2);
+            const double num = na * ns + nj2;
+            const double den = nj * nc + ns2;
+            cand = dt_bits(sqrt(eta * num / den));
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) f0.c[c][i] = f1.c[c][i];
+    }
+    if (dt_next) block_min_u64(cand, dt_next);
+}
+
+__global__ void k_dt_advance(double* dt, unsigned long long* dt_next)
+{
+    if (threadIdx.x == 0) {
+        *dt = __longlong_as_double((long long)*dt_next);
+        *dt_next = (unsigned long long)__double_as_longlong(INFINITY);
+    }
+}
+
+__global__ void k_fill_inf(unsigned long long* p)
+{
+    if (threadIdx.x == 0) *p = (unsigned long long)__double_as_longlong(INFINITY);
+}
+
+// NBody::initialdt, src/nbody.cpp:131-150.
+__global__ __launch_bounds__(kOBlock) void k_initial_dt(int n, Forces f, double eta0,
+                                                        unsigned long long* out)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kOBlock + threadIdx.x;
+    unsigned long long cand = ~0ull;
+    if (i < n) {
+        const double a2x = f.c[0][i] * f.c[0][i];
+        const double a2y = f.c[1][i] * f.c[1][i];
+        const double a2z = f.c[2][i] * f.c[2][i];
+        const double an = sqrt(a2x + a2y + a2z);
+        const double j2x = f.c[3][i] * f.c[3][i];
+        const double j2y = f.c[4][i] * f.c[4][i];
+        const double j2z = f.c[5][i] * f.c[5][i];
+        const double jn = sqrt(j2x + j2y + j2z);
+        cand = dt_bits(eta0 * an / jn);
+    }
+    block_min_u64(cand, out);
+}
+
+// ----------------------------------------------------------- energy -----
+// Kinetic and potential energy in FP64 (src/utils.cpp:125-153).  Per target
+// body i the potential sum phi_i runs over source tiles like the pair
+// kernels.  compat = 0: U = -1/2 sum_i m_i sum_{j != i} m_j / sqrt(r^2+eps2).
+// compat = 1: the reference's own definition, U = -sum_i sum_{j>i} m_j^2 /
+// sqrt(r^2 + eps2) (src/utils.cpp:148).  Block sums are written per block
+// and added in block order by k_energy_final: deterministic.
+constexpr int kEBlock = 256;
+
+template <bool COMPAT>
+__global__ __launch_bounds__(kEBlock) void k_energy(State s, int n, double eps2, double* work)
+{
+    __shared__ double4 tile[kEBlock];
+    __shared__ double red[2][kEBlock / 64];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * kEBlock + t;
+    const int ic = i < n ? i : n - 1;
+    const double xi = s.x[ic], yi = s.y[ic], zi = s.z[ic];
+    double phi = 0.0;
+    for (int jt = 0; jt < n; jt += kEBlock) {
+        const int j = jt + t;
+        double4 b = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (j < n) b = make_double4(s.x[j], s.y[j], s.z[j], s.m[j]);
+        __syncthreads();
+        tile[t] = b;
+        __syncthreads();
+        const int cnt = min(kEBlock, n - jt);
+        for (int k = 0; k < cnt; ++k) {
+            const double4 q = tile[k];
+            const int jj = jt + k;
+            const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+            double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+            const bool keep = COMPAT ? (jj > i) : (jj != i);
+            r2 = keep ? r2 : 1.0;
+            const double w = keep ? (COMPAT ? q.w * q.w : q.w) : 0.0;
+            phi = fma(w, rsqrt_f64(r2), phi);
+        }
+    }
+    double pot = 0.0, kin = 0.0;
+    if (i < n) {
+        const double mi = s.m[i];
+        pot = COMPAT ? -phi : -0.5 * mi * phi;
+        const double ux = s.vx[i], uy = s.vy[i], uz = s.vz[i];
+        kin = 0.5 * mi * (ux * ux + uy * uy + uz * uz);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        pot += __shfl_down(pot, off, 64);
+        kin += __shfl_down(kin, off, 64);
+    }
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = kin;
+        red[1][t >> 6] = pot;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double k = 0.0, p = 0.0;
+        for (int w = 0; w < kEBlock / 64; ++w) {
+            k += red[0][w];
+            p += red[1][w];
+        }
+        work[2 * blockIdx.x] = k;
+        work[2 * blockIdx.x + 1] = p;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy_final(const double* work, int blocks, double* out)
+{
+    __shared__ double rk[256], rp[256];
+    const int t = threadIdx.x;
+    double k = 0.0, p = 0.0;
+    for (int b = t; b < blocks; b += 256) {
+        k += work[2 * b];
+        p += work[2 * b + 1];
+    }
+    rk[t] = k;
+    rp[t] = p;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            rk[t] += rk[t + h];
+            rp[t] += rp[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = rk[0];
+        out[1] = rp[0];
+    }
+}
+
+__global__ __launch_bounds__(kOBlock) void k_check_finite(int n, Forces f, int* flag)
+{
+    const int i = blockIdx.x * kOBlock + threadIdx.x;
+    if (i >= n) return;
+    bool bad = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) bad |= !isfinite(f.c[c][i]);
+    if (bad) atomicOr(flag, 1);
+}
+
+inline int oblocks(int n) { return (n + kOBlock - 1) / kOBlock; }
+
+template <int BLOCK>
+hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, hipStream_t st)
+{
+    const dim3 grid((a.n + BLOCK - 1) / BLOCK, jsplit);
+    if (precision == 0) {
+        if (mask)
+            k_force_f64<BLOCK, true><<<grid, BLOCK, 0, st>>>(a);
+        else
+            k_force_f64<BLOCK, false><<<grid, BLOCK, 0, st>>>(a);
+    } else {
+        if (mask)
+            k_force_mixed<BLOCK, true><<<grid, BLOCK, 0, st>>>(a);
+        else
+            k_force_mixed<BLOCK, false><<<grid, BLOCK, 0, st>>>(a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_force(int precision, const State& s, int n, double eps2,
+                        const Shape& shape, const Forces& out, double* partial,
+                        hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+{
+    PairArgs a;
+    a.s = s;
+    a.n = n;
+    a.jchunk = shape.jchunk;
+    a.eps2 = eps2;
+    if (shape.jsplit > 1) {
+        // partial plane for (component c, split k) starts at (c*S + k)*n
+        for (int c = 0; c < 6; ++c) a.out.c[c] = partial + (size_t)c * shape.jsplit * n;
+        a.split_stride = (size_t)n;
+    } else {
+        a.out = out;
+        a.split_stride = 0;
+    }
+    const bool mask = (eps2 == 0.0);
+    hipError_t e;
+    if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
+    switch (shape.block) {
+    case 64: e = launch_pair<64>(precision, mask, a, shape.jsplit, st); break;
+    case 128: e = launch_pair<128>(precision, mask, a, shape.jsplit, st); break;
+    case 256: e = launch_pair<256>(precision, mask, a, shape.jsplit, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
+    if (shape.jsplit > 1) {
+        k_reduce_splits<<<dim3(oblocks(n), 6), kOBlock, 0, st>>>(partial, n, shape.jsplit, out);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t launch_predict(int n, const double* dt, double* x, double* y, double* z,
+                          double* vx, double* vy, double* vz, const Forces& f0,
+                          hipStream_t st)
+{
+    k_predict<<<oblocks(n), kOBlock, 0, st>>>(n, dt, x, y, z, vx, vy, vz, f0);
+    return hipGetLastError();
+}
+
+hipError_t launch_correct(int n, const double* dt, double* x, double* y, double* z,
+                          double* vx, double* vy, double* vz, const Forces& f0,
+                          const Forces& f1, unsigned long long* dt_next, double eta,
+                          hipStream_t st)
+{
+    k_correct<<<oblocks(n), kOBlock, 0, st>>>(n, dt, x, y, z, vx, vy, vz, f0, f1, dt_next, eta);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_advance(double* dt, unsigned long long* dt_next, hipStream_t st)
+{
+    k_dt_advance<<<1, 64, 0, st>>>(dt, dt_next);
+    return hipGetLastError();
+}
+
+hipError_t launch_initial_dt(int n, const Forces& f, double eta0, unsigned long long* out,
+                             hipStream_t st)
+{
+    k_initial_dt<<<oblocks(n), kOBlock, 0, st>>>(n, f, eta0, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_inf(unsigned long long* p, hipStream_t st)
+{
+    k_fill_inf<<<1, 64, 0, st>>>(p);
+    return hipGetLastError();
+}
+
+int energy_blocks(int n) { return (n + kEBlock - 1) / kEBlock; }
+
+hipError_t launch_energy(const State& s, int n, double eps2, int compat, double* work,
+                         double* out, hipStream_t st)
+{
+    const int nb = energy_blocks(n);
+    if (compat)
+        k_energy<true><<<nb, kEBlock, 0, st>>>(s, n, eps2, work);
+    else
+        k_energy<false><<<nb, kEBlock, 0, st>>>(s, n, eps2, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_energy_final<<<1, 256, 0, st>>>(work, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_finite(int n, const Forces& f, int* flag, hipStream_t st)
+{
+    k_check_finite<<<oblocks(n), kOBlock, 0, st>>>(n, f, flag);
+    return hipGetLastError();
+}
+
+}  // namespace nbh

@@ -1,0 +1,230 @@
+"""Python handle onto libnbody_hip.so (the C ABI of include/nbody_hip.h).
+
+Thin ctypes layer for tests, the benchmark and Python callers.  Every call
+goes to the HIP library; there is no CPU fallback: if the library is not
+built, importing succeeds but any use raises `BackendMissing`.
+
+Arrays are numpy float64.  A body state is a (7, n) array in the order of
+the reference's force_calculation arguments (include/kernel.h:15-19):
+x, y, z, m, vx, vy, vz.  Forces are (6, n): ax, ay, az, jx, jy, jz.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(ROOT, "lib", "libnbody_hip.so")
+HEADER = os.path.join(os.path.dirname(ROOT), "include", "nbody_hip.h")
+
+FP64 = 0
+MIXED = 1
+PRECISIONS = {"fp64": FP64, "mixed": MIXED}
+
+OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_NONFINITE = range(6)
+
+_D = C.POINTER(C.c_double)
+
+
+class BackendMissing(RuntimeError):
+    pass
+
+
+class NBodyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("eps2", C.c_double),
+                ("block", C.c_int), ("jsplit", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library once; raises BackendMissing if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BackendMissing(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C n-body-code_amd)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.nbh_last_error.restype = C.c_char_p
+    L.nbh_version.restype = C.c_int
+    L.nbh_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.nbh_destroy.argtypes = [vp]
+    L.nbh_configure.argtypes = [vp, C.POINTER(Config)]
+    L.nbh_force.argtypes = [vp, C.c_int] + [_D] * 13 + [_D]
+    L.nbh_force_device.argtypes = [vp, C.c_int] + [vp] * 13 + [vp]
+    L.nbh_upload.argtypes = [vp, C.c_int] + [_D] * 7
+    L.nbh_initial_dt.argtypes = [vp, C.c_double, _D]
+    L.nbh_set_dt.argtypes = [vp, C.c_double, C.c_int, C.c_double]
+    L.nbh_get_dt.argtypes = [vp, _D]
+    L.nbh_step.argtypes = [vp, C.c_int, C.c_int]
+    L.nbh_download.argtypes = [vp] + [_D] * 13
+    L.nbh_energy.argtypes = [vp, C.c_int, _D, _D]
+    L.nbh_energy_host.argtypes = [vp, C.c_int] + [_D] * 7 + [C.c_int, _D, _D]
+    L.nbh_set_profiling.argtypes = [vp, C.c_int]
+    L.nbh_timing.argtypes = [vp, _D, C.POINTER(C.c_int64), _D, C.POINTER(C.c_int64)]
+    L.nbh_reset_timing.argtypes = [vp]
+    L.nbh_launch_shape.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.nbh_device_info.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
+    L.nbh_plummer.argtypes = [C.c_int, C.c_uint64] + [_D] * 7
+    L.nbh_reference_ic.argtypes = [C.c_int] + [C.c_double] * 4 + [_D] * 7
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != OK:
+        raise NBodyError(rc, lib().nbh_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _state(state):
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    if s.ndim != 2 or s.shape[0] != 7:
+        raise ValueError(f"state must be (7, n), got {s.shape}")
+    return s
+
+
+def plummer(n, seed):
+    """Equal-mass Plummer sphere in Henon units (host code); (7, n)."""
+    s = np.empty((7, n))
+    rc = lib().nbh_plummer(int(n), int(seed), *[_p(s[k]) for k in range(7)])
+    if rc != OK:
+        raise NBodyError(rc, f"nbh_plummer(n={n}) rejected its arguments")
+    return s
+
+
+def reference_ic(n, vmax=1.0, rmax=1.0, mmin=0.01, mmax=1.0):
+    """NBody::initialize of the reference CLI (src/nbody.cpp:69-114); (7, n)."""
+    s = np.empty((7, n))
+    rc = lib().nbh_reference_ic(int(n), vmax, rmax, mmin, mmax, *[_p(s[k]) for k in range(7)])
+    if rc != OK:
+        raise NBodyError(rc, f"nbh_reference_ic(n={n}) rejected its arguments")
+    return s
+
+
+def device_info(device=0):
+    cu, ws = C.c_int(), C.c_int()
+    name = C.create_string_buffer(64)
+    _check(lib().nbh_device_info(device, C.byref(cu), C.byref(ws), name, 64))
+    return {"compute_units": cu.value, "wave_size": ws.value, "arch": name.value.decode()}
+
+
+class Backend:
+    """One nbh_ctx: a device, a stream, buffers, a resident state."""
+
+    def __init__(self, precision="fp64", eps2=1e-7, device=0, block=0, jsplit=0):
+        L = lib()
+        self._cfg = Config(device, PRECISIONS[precision] if isinstance(precision, str) else int(precision),
+                           float(eps2), int(block), int(jsplit))
+        h = C.c_void_p()
+        _check(L.nbh_create(C.byref(self._cfg), C.byref(h)))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nbh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def configure(self, precision=None, eps2=None, block=None, jsplit=None):
+        c = self._cfg
+        if precision is not None:
+            c.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        if eps2 is not None:
+            c.eps2 = float(eps2)
+        if block is not None:
+            c.block = int(block)
+        if jsplit is not None:
+            c.jsplit = int(jsplit)
+        _check(lib().nbh_configure(self._h, C.byref(c)))
+
+    # ---- drop-in force (force_calculation) ----
+    def force(self, state, return_ms=False):
+        s = _state(state)
+        n = s.shape[1]
+        out = np.empty((6, n))
+        ms = C.c_double()
+        _check(lib().nbh_force(self._h, n, *[_p(s[k]) for k in range(7)],
+                               *[_p(out[k]) for k in range(6)], C.byref(ms)))
+        return (out, ms.value) if return_ms else out
+
+    def launch_shape(self, n):
+        b, s, w = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().nbh_launch_shape(self._h, int(n), C.byref(b), C.byref(s), C.byref(w)))
+        return {"block": b.value, "jsplit": s.value, "workgroups": w.value}
+
+    # ---- resident integrator ----
+    def upload(self, state):
+        s = _state(state)
+        self.n = s.shape[1]
+        _check(lib().nbh_upload(self._h, self.n, *[_p(s[k]) for k in range(7)]))
+
+    def initial_dt(self, eta0=1e-4):
+        d = C.c_double()
+        _check(lib().nbh_initial_dt(self._h, float(eta0), C.byref(d)))
+        return d.value
+
+    def set_dt(self, dt, adaptive=False, eta=1e-2):
+        _check(lib().nbh_set_dt(self._h, float(dt), int(bool(adaptive)), float(eta)))
+
+    def get_dt(self):
+        d = C.c_double()
+        _check(lib().nbh_get_dt(self._h, C.byref(d)))
+        return d.value
+
+    def step(self, nsteps=1, graph=False):
+        _check(lib().nbh_step(self._h, int(nsteps), int(bool(graph))))
+
+    def download(self):
+        s = np.empty((7, self.n))
+        f = np.empty((6, self.n))
+        _check(lib().nbh_download(self._h, *[_p(s[k]) for k in range(7)], *[_p(f[k]) for k in range(6)]))
+        return s, f
+
+    def energy(self, compat=False):
+        k, u = C.c_double(), C.c_double()
+        _check(lib().nbh_energy(self._h, int(bool(compat)), C.byref(k), C.byref(u)))
+        return k.value, u.value
+
+    def energy_of(self, state, compat=False):
+        s = _state(state)
+        k, u = C.c_double(), C.c_double()
+        _check(lib().nbh_energy_host(self._h, s.shape[1], *[_p(s[i]) for i in range(7)],
+                                     int(bool(compat)), C.byref(k), C.byref(u)))
+        return k.value, u.value
+
+    # ---- timing ----
+    def set_profiling(self, on=True):
+        _check(lib().nbh_set_profiling(self._h, int(bool(on))))
+
+    def timing(self):
+        fm, fl, sm, st = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+        _check(lib().nbh_timing(self._h, C.byref(fm), C.byref(fl), C.byref(sm), C.byref(st)))
+        return {"force_ms": fm.value, "force_launches": fl.value, "step_ms": sm.value, "steps": st.value}
+
+    def reset_timing(self):
+        _check(lib().nbh_reset_timing(self._h))
