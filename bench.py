@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: pair-interactions/s of the Hermite-4 step on one MI355X.
+
+Workload: N = 1,048,576 equal-mass Plummer sphere (seed 3) in Henon units,
+eps = 1e-3, dt = 1/2048, resident in HBM.  A step is one shared-timestep
+Hermite step (predict, pair force + jerk over all N^2 directed pairs,
+correct), run by the device-resident integrator.  Both precisions run:
+  FP64  -- the drop-in (plain FP64 pair arithmetic);
+  mixed -- FP64 position differences rounded to FP32, FP32 r^2 / rsqrt /
+           jerk, two-sum compensated FP32 sums (the headline `value`).
+`value` is the mixed path's N^2 * steps / time.  The mixed path is valid
+only where its median per-body relative acceleration error against FP64 is
+<= 1e-6; that error is measured in the same run on the same state.
+
+Roofline: the pair kernel is VALU-bound (no MFMA; ~6e5 flop per HBM byte).
+achieved = 60 flop per directed pair (acc + jerk convention) * N^2 / the
+kernel's mean launch time from HIP events recorded on its own stream in the
+timed region; peak = MI355X vector peak (FP32 157.3 TFLOP/s for the mixed
+kernel, FP64 78.6 TFLOP/s for the FP64 kernel).
+
+cpu_baseline: the oracle's restatement of the reference FP64 force loop
+(src/kernel.cpp:1164-1224: Newton-3rd-law triangle, cyclic rows per thread,
+per-thread sums merged at the end) on the first M bodies of the same
+sphere, OpenMP threads on this host's cores, M sized for ~10-20 s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "n-body-code_amd"))
+
+N = 1 << 20
+SEED = 3
+EPS = 1e-3
+DT = 1.0 / 2048
+FLOP_PER_PAIR = 60
+PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector
+
+
+def body_relerr(a, b):
+    return np.sqrt(((a - b) ** 2).sum(0)) / np.sqrt((b ** 2).sum(0))
+
+
+def run_precision(nb, prec, state, steps, warmup, log):
+    with nb.Backend(prec, EPS * EPS) as b:
+        b.upload(state)                 # initial force, NBody::cycle
+        f0 = b.download()[1]
+        b.set_dt(DT)
+        if warmup:
+            b.step(warmup)
+        e0 = sum(b.energy())
+        b.reset_timing()
+        b.set_profiling(True)
+        t0 = time.perf_counter()
+        b.step(steps)                   # synchronous: stream drained on return
+        t1 = time.perf_counter()
+        b.set_profiling(False)
+        tm = b.timing()
+        e1 = sum(b.energy())
+        shape = b.launch_shape(N)
+    sec = t1 - t0
+    kms = tm["force_ms"] / max(tm["force_launches"], 1)
+    res = {
+        "pairs_per_s": N * N * steps / sec,
+        "ms_per_step": 1e3 * sec / steps,
+        "steps_per_s": steps / sec,
+        "kernel_ms": kms,
+        "kernel_launches": tm["force_launches"],
+        "kernel_tflops": FLOP_PER_PAIR * N * N / (kms * 1e-3) / 1e12,
+        "energy_drift": abs(e1 - e0) / abs(e0),
+        "shape": shape,
+    }
+    log(f"{prec}: {res['ms_per_step']:.1f} ms/step, kernel {kms:.1f} ms, "
+        f"{res['pairs_per_s']:.3e} pairs/s, drift {res['energy_drift']:.2e}")
+    return res, f0
+
+
+def cpu_baseline(state, budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    O = Oracle()
+    cores = max(1, min(16, os.cpu_count() or 1))
+    m0 = 8192
+    t = time.perf_counter()
+    O.force(state[:, :m0], EPS * EPS, "fp64", nthreads=cores)
+    rate = m0 * (m0 - 1) / (time.perf_counter() - t)
+    m = int(np.sqrt(rate * budget_s))
+    m = int(min(max(m, m0), 262144)) // 1024 * 1024
+    t = time.perf_counter()
+    O.force(state[:, :m], EPS * EPS, "fp64", nthreads=cores)
+    sec = time.perf_counter() - t
+    return {"value": m * (m - 1) / sec, "unit": "pair-interactions/s", "cores": cores,
+            "kind": "port",
+            "sample": f"one FP64 force evaluation (reference triangle order) over the first {m} bodies "
+                      f"of the N=1048576 seed-3 Plummer sphere, eps=1e-3, {cores} OpenMP threads, "
+                      f"{sec:.1f} s; rate = M(M-1)/t directed pairs"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU split is design-only).", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+    log = (lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)) if rank == 0 else (lambda m: None)
+
+    import nbody_amd as nb
+    log(f"device {nb.device_info()}")
+    t = time.perf_counter()
+    state = nb.plummer(N, SEED)
+    log(f"Plummer N={N} in {time.perf_counter() - t:.1f} s")
+
+    r64, f64 = run_precision(nb, "fp64", state, args.steps, args.warmup, log)
+    rmx, fmx = run_precision(nb, "mixed", state, args.steps, args.warmup, log)
+    err = body_relerr(fmx[:3], f64[:3])
+    med, mx = float(np.median(err)), float(err.max())
+    log(f"mixed vs fp64 acc rel err: median {med:.2e}, max {mx:.2e}")
+
+    cpu = None
+    if not args.no_cpu_baseline and rank == 0:
+        cpu = cpu_baseline(state)
+        log(f"cpu baseline {cpu['value']:.3e} pairs/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "pair-interactions/s at N=1M: mixed vs FP64 GPU @ median rel acc err <=1e-6",
+        "value": rmx["pairs_per_s"],
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": rmx["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64/f32 mixed",
+        "data": "synthetic (Plummer sphere, generated in-process)",
+        "config": {"workload": "Hermite-4 shared step, N=1048576 Plummer seed 3, eps=1e-3, dt=1/2048",
+                   "n_bodies": N, "global_batch": N, "seq_len": None, "parallelism": "none (1 GPU)"},
+        "valid": med <= 1e-6,
+        "mixed_median_rel_acc_err": med,
+        "mixed_max_rel_acc_err": mx,
+        "fp64_pairs_per_s": r64["pairs_per_s"],
+        "mixed_over_fp64": rmx["pairs_per_s"] / r64["pairs_per_s"],
+        "steps_per_s": {"fp64": r64["steps_per_s"], "mixed": rmx["steps_per_s"]},
+        "energy_drift": {"fp64": r64["energy_drift"], "mixed": rmx["energy_drift"]},
+        "launch_shape": rmx["shape"],
+        "roofline": {"bound": "valu", "achieved": rmx["kernel_tflops"], "peak": PEAK["mixed"],
+                     "unit": "TFLOP/s", "frac": rmx["kernel_tflops"] / PEAK["mixed"], "traffic": None,
+                     "kernel": "k_force_mixed<256>", "kernel_ms": rmx["kernel_ms"],
+                     "flop_per_launch": FLOP_PER_PAIR * N * N},
+        "roofline_fp64": {"bound": "valu", "achieved": r64["kernel_tflops"], "peak": PEAK["fp64"],
+                          "unit": "TFLOP/s", "frac": r64["kernel_tflops"] / PEAK["fp64"], "traffic": None,
+                          "kernel": "k_force_f64<256>", "kernel_ms": r64["kernel_ms"],
+                          "flop_per_launch": FLOP_PER_PAIR * N * N},
+        "cpu_baseline": cpu,
+    }
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

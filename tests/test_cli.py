@@ -1,0 +1,64 @@
+"""nbody_app: the reference CLI (src/main.cpp:13-143) on the HIP backend."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT, golden
+
+APP = os.path.join(ROOT, "n-body-code_amd", "bin", "nbody_app")
+
+
+def run(*args, timeout=120):
+    if not os.path.exists(APP):
+        pytest.fail(f"{APP} is not built")
+    return subprocess.run([APP, *args], capture_output=True, text=True, timeout=timeout)
+
+
+@pytest.mark.parametrize("args,needle", [
+    (["--N", "0"], "N must be positive"),
+    (["--N", "abc"], "Error parsing --N"),
+    (["--Ncycle", "-3"], "Ncycle must be positive"),
+    (["--bogus"], "Unknown or incomplete argument"),
+    (["--N"], "Unknown or incomplete argument"),
+    (["--precision", "fp16"], "precision must be"),
+    (["--dt", "0"], "dt must be positive"),
+])
+def test_bad_arguments_exit_nonzero(args, needle):
+    r = run(*args)
+    assert r.returncode == 1
+    assert needle in r.stderr
+
+
+def energies(out):
+    e0 = float(re.search(r"Initial Energy: (\S+)", out).group(1))
+    e1 = float(re.search(r"Final Energy: (\S+)", out).group(1))
+    return e0, e1
+
+
+@pytest.mark.gpu
+def test_reference_run_initial_energy_matches_reference():
+    """Default flags are the reference's: its IC, EPS2 = 1e-7, adaptive dt,
+    its energy definition; the initial energy is the reference's E0."""
+    g = golden("ref_ic_n1024.npz")
+    r = run("--N", "1024", "--Ncycle", "3")
+    assert r.returncode == 0, r.stderr
+    e0, e1 = energies(r.stdout)
+    ref = float(g["kinetic"]) + float(g["potential"])
+    assert abs(e0 - ref) <= 1e-9 * abs(ref)   # printed with 9 decimals
+    assert "Total active cores: 256" in r.stdout
+    rows = [l for l in r.stdout.splitlines() if re.match(r"\s+1024\s+1\s", l)]
+    assert len(rows) == 3
+
+
+@pytest.mark.gpu
+def test_resident_matches_dropin():
+    common = ["--N", "2048", "--Ncycle", "8", "--ic", "plummer", "--seed", "4", "--eps", "0.01",
+              "--dt", "0.001", "--energy", "physical"]
+    a = run(*common)
+    b = run(*common, "--resident", "--graph")
+    assert a.returncode == 0 and b.returncode == 0, a.stderr + b.stderr
+    ea, eb = energies(a.stdout), energies(b.stdout)
+    assert ea[0] == eb[0]
+    assert abs(ea[1] - eb[1]) <= 1e-12 * abs(ea[1]) + 1e-9

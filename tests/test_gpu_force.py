@@ -1,0 +1,183 @@
+"""GPU parity of the pair kernels against the oracle and the reference's
+golden vectors, through the C ABI (libnbody_hip.so).
+
+Bars: FP64 per-body |da|/|a| and |dj|/|j| <= 1e-12 (summation order differs
+from the reference's j<i triangle, so not bitwise); mixed per-body median
+relative acceleration error <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+from conftest import body_relerr, golden
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+EPS2_REF = 1e-7
+
+
+def _banner(msg):
+    print(f"[gpu-force] {msg}", flush=True)
+
+
+def check64(f, ref, tol=TOL64):
+    ea = body_relerr(f[:3], ref[:3])
+    ej = body_relerr(f[3:], ref[3:])
+    assert ea.max() <= tol, f"acc max rel err {ea.max():.3e}"
+    assert ej.max() <= tol, f"jerk max rel err {ej.max():.3e}"
+    return ea.max(), ej.max()
+
+
+@pytest.fixture(scope="module")
+def be64(nb):
+    with nb.Backend("fp64", EPS2_REF) as b:
+        yield b
+
+
+@pytest.fixture(scope="module")
+def bemx(nb):
+    with nb.Backend("mixed", EPS2_REF) as b:
+        yield b
+
+
+def test_fp64_vs_reference_fixtures(be64):
+    for name in ("ref_ic_n1024.npz", "plummer_n1024.npz"):
+        _banner(f"fixture {name}")
+        g = golden(name)
+        be64.configure(eps2=EPS2_REF)
+        check64(be64.force(g["state"]), g["force64"])
+
+
+def test_fp64_plummer_config1_vs_oracle(be64, oracle, nb):
+    """N=1024 Plummer seed 1, eps = 1/64 (the softening is a parameter)."""
+    s = nb.plummer(1024, 1)
+    eps2 = (1.0 / 64) ** 2
+    be64.configure(eps2=eps2)
+    ea, ej = check64(be64.force(s), oracle.force(s, eps2))
+    _banner(f"config1 fp64 acc {ea:.2e} jerk {ej:.2e}")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 65, 257, 1000, 1023, 1025, 4097])
+def test_fp64_ragged_sizes(be64, oracle, nb, n):
+    _banner(f"ragged n={n}")
+    s = nb.plummer(n, 1000 + n)
+    be64.configure(eps2=1e-4, block=0, jsplit=0)
+    f = be64.force(s)
+    ref = oracle.force(s, 1e-4)
+    if n == 1:
+        assert np.all(f == 0.0)
+    else:
+        check64(f, ref)
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("jsplit", [1, 2, 7, 0])
+def test_fp64_launch_shapes_agree(be64, oracle, nb, block, jsplit):
+    _banner(f"shape block={block} jsplit={jsplit}")
+    s = nb.plummer(3000, 77)
+    be64.configure(eps2=1e-3, block=block, jsplit=jsplit)
+    try:
+        check64(be64.force(s), oracle.force(s, 1e-3))
+    finally:
+        be64.configure(block=0, jsplit=0)
+
+
+def test_fp64_deterministic(be64, nb):
+    s = nb.plummer(5000, 3)
+    be64.configure(eps2=1e-4, block=0, jsplit=0)
+    assert be64.launch_shape(5000)["jsplit"] > 1
+    assert np.array_equal(be64.force(s), be64.force(s))
+
+
+def test_fp64_zero_softening(be64, oracle, nb):
+    """eps2 = 0: the self pair is dropped, distinct bodies interact unsoftened."""
+    s = nb.plummer(777, 8)
+    be64.configure(eps2=0.0)
+    check64(be64.force(s), oracle.force(s, 0.0))
+
+
+def test_coincident_bodies_unsoftened_reported(be64, nb):
+    s = nb.plummer(100, 9)
+    s[0:3, 5] = s[0:3, 6]
+    be64.configure(eps2=0.0)
+    with pytest.raises(nb.NBodyError) as e:
+        be64.force(s)
+    assert e.value.code == nb.ERR_NONFINITE
+    be64.configure(eps2=1e-4)
+    be64.force(s)  # softened: finite
+
+
+def test_mixed_vs_oracle(bemx, oracle, nb):
+    s = nb.plummer(1024, 1)
+    eps2 = (1.0 / 64) ** 2
+    bemx.configure(eps2=eps2)
+    f = bemx.force(s)
+    ref = oracle.force(s, eps2)
+    ea = body_relerr(f[:3], ref[:3])
+    ej = body_relerr(f[3:], ref[3:])
+    _banner(f"mixed acc median {np.median(ea):.2e} max {ea.max():.2e}; jerk median {np.median(ej):.2e}")
+    assert np.median(ea) <= 1e-6
+    assert ea.max() <= 1e-5
+    assert np.median(ej) <= 1e-5
+
+
+def test_mixed_vs_reference_f32_fixture(bemx):
+    """The reference's _F32 build is FP32 arithmetic with double sums; the
+    mixed kernel must sit within FP32 rounding of it."""
+    g = golden("plummer_n1024.npz")
+    bemx.configure(eps2=EPS2_REF)
+    f = bemx.force(g["state"])
+    e = body_relerr(f[:3], g["force32"][:3])
+    assert np.median(e) <= 1e-6
+
+
+def test_large_sample_vs_reference_fixture(be64, bemx, nb):
+    """N=102400 Plummer seed 2 (the 102,400-body configuration's sphere) at the
+    reference's softening, every 100th body against the reference."""
+    g = golden("plummer_n102400_sample.npz")
+    s = nb.plummer(int(g["n"]), int(g["seed"]))
+    idx = g["idx"]
+    be64.configure(eps2=EPS2_REF)
+    f = be64.force(s)
+    check64(f[:, idx], g["force64"])
+    bemx.configure(eps2=EPS2_REF)
+    fm = bemx.force(s)
+    e = body_relerr(fm[:3, idx], g["force64"][:3])
+    _banner(f"N=102400 mixed median {np.median(e):.2e}")
+    assert np.median(e) <= 1e-6
+
+
+def test_million_bodies_properties(be64, bemx, oracle, nb):
+    """N = 1,048,576 Plummer seed 3, eps = 1e-3: a sample of bodies against the
+    long-double directed sum, momentum conservation, mixed vs FP64."""
+    n = 1 << 20
+    s = nb.plummer(n, 3)
+    be64.configure(eps2=1e-6)
+    f = be64.force(s)
+    idx = np.random.default_rng(0).choice(n, 24, replace=False)
+    ref = oracle.force_sample(s, 1e-6, idx, nthreads=16)
+    check64(f[:, idx], ref)
+    m = s[3]
+    for c in range(6):
+        assert abs(np.sum(m * f[c])) <= 1e-12 * np.sum(np.abs(m * f[c]))
+    bemx.configure(eps2=1e-6)
+    fm = bemx.force(s)
+    e = body_relerr(fm[:3], f[:3])
+    _banner(f"N=1M mixed vs fp64 acc median {np.median(e):.2e} max {e.max():.2e}")
+    assert np.median(e) <= 1e-6
+
+
+def test_force_device_pointers(be64, oracle, nb):
+    """nbh_force_device on torch-allocated HBM buffers."""
+    import torch
+    s = nb.plummer(2048, 21)
+    d = torch.from_numpy(s).to("cuda")
+    out = torch.empty((6, 2048), dtype=torch.float64, device="cuda")
+    be64.configure(eps2=1e-4)
+    ptr = lambda t: t.data_ptr()  # noqa: E731
+    torch.cuda.synchronize()
+    rc = nb.lib().nbh_force_device(be64._h, 2048, *[ptr(d[k]) for k in range(7)],
+                                   *[ptr(out[k]) for k in range(6)], None)
+    assert rc == nb.OK
+    be64.timing()  # drains the context stream
+    check64(out.cpu().numpy(), oracle.force(s, 1e-4))

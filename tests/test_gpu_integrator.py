@@ -1,0 +1,170 @@
+"""GPU parity of the device-resident Hermite integrator (predictor, force,
+corrector, adaptive dt, energy, hipGraph replay) against the oracle and the
+reference's golden trajectories, through the C ABI."""
+import numpy as np
+import pytest
+
+from conftest import body_relerr, golden
+
+pytestmark = pytest.mark.gpu
+
+EPS2_REF = 1e-7
+
+
+def _banner(msg):
+    print(f"[gpu-integrator] {msg}", flush=True)
+
+
+def oracle_trajectory(oracle, s, eps2, dt, nsteps):
+    """Reference step order (src/nbody.cpp:224-289); yields (state, force)
+    after the initial evaluation and after every step."""
+    f0 = oracle.force(s, eps2)
+    yield s, f0
+    for _ in range(nsteps):
+        p = oracle.predict(s, f0, dt)
+        f1 = oracle.force(p, eps2)
+        s, _, _ = oracle.snap_correct(p, f0, f1, dt)
+        f0 = f1
+        yield s, f0
+
+
+def test_config1_energy_every_step(nb, oracle):
+    """N=1024 Plummer seed 1, eps = 1/64, dt = 1/512, 256 FP64 steps.
+    Gates: per-body a and jerk at the first two evaluations <= 1e-12; total
+    energy at every step within 1e-12 relative of the oracle's.  The final
+    per-body divergence is reported, not gated (the orbits are chaotic)."""
+    s0 = nb.plummer(1024, 1)
+    eps2, dt, nsteps = (1 / 64) ** 2, 1 / 512, 256
+    traj = oracle_trajectory(oracle, s0, eps2, dt, nsteps)
+    with nb.Backend("fp64", eps2) as b:
+        b.upload(s0)
+        b.set_dt(dt)
+        worst_e = 0.0
+        for k, (so, fo) in enumerate(traj):
+            if k > 0:
+                b.step(1)
+            st, f = b.download()
+            if k <= 1:
+                assert body_relerr(f[:3], fo[:3]).max() <= 1e-12
+                assert body_relerr(f[3:], fo[3:]).max() <= 1e-12
+            kin, pot = b.energy()
+            e_gpu = kin + pot
+            e_or = oracle.kinetic(so) + oracle.potential(so, eps2, nthreads=16)
+            worst_e = max(worst_e, abs(e_gpu - e_or) / abs(e_or))
+        assert worst_e <= 1e-12, worst_e
+        div = np.abs(st[0:3] - so[0:3]).max()
+        _banner(f"config1: worst energy rel diff {worst_e:.2e}, final position divergence {div:.2e}")
+        assert div < 1e-6
+
+
+def test_fixed_dt_vs_reference_trajectory(nb):
+    g = golden("plummer_n1024.npz")
+    with nb.Backend("fp64", EPS2_REF) as b:
+        b.upload(g["state"])
+        b.set_dt(float(g["fixed_dt"]))
+        b.step(8)
+        st, f = b.download()
+    assert np.abs(st - g["fixed_state"]).max() <= 1e-13
+    assert body_relerr(f[:3], g["fixed_force"][:3]).max() <= 1e-10
+
+
+def test_adaptive_dt_vs_reference_trajectory(nb):
+    """NBody::initialdt then updatedt every step (src/nbody.cpp:131-222)."""
+    g = golden("plummer_n1024.npz")
+    with nb.Backend("fp64", EPS2_REF) as b:
+        b.upload(g["state"])
+        dts = [b.initial_dt(1e-4)]
+        b.set_dt(dts[0], adaptive=True, eta=1e-2)
+        for _ in range(8):
+            b.step(1)
+            dts.append(b.get_dt())
+        st, _ = b.download()
+    assert np.allclose(dts, g["adapt_dts"], rtol=1e-10, atol=0)
+    assert np.abs(st - g["adapt_state"]).max() <= 1e-12
+
+
+@pytest.mark.parametrize("n", [2, 3, 8, 64])
+def test_tiny_systems_vs_reference(nb, n):
+    g = golden("tiny.npz")
+    with nb.Backend("fp64", EPS2_REF) as b:
+        b.upload(g[f"state{n}"])
+        b.set_dt(float(g["dt"]))
+        b.step(4)
+        st, f = b.download()
+    assert np.abs(st - g[f"final{n}"]).max() <= 1e-13
+    assert body_relerr(f[:3], g[f"force{n}"][:3]).max() <= 1e-12
+
+
+def test_graph_replay_matches_eager(nb):
+    s = nb.plummer(2000, 5)
+    out = []
+    for graph in (False, True, True):
+        with nb.Backend("fp64", 1e-4) as b:
+            b.upload(s)
+            b.set_dt(1 / 1024)
+            b.step(40, graph=graph)   # one 32-step graph + 8 eager steps
+            b.step(64, graph=graph)   # the graph replayed twice
+            out.append(b.download())
+    for st, f in out[1:]:
+        assert np.array_equal(st, out[0][0])
+        assert np.array_equal(f, out[0][1])
+
+
+def test_adaptive_graph_matches_eager(nb):
+    s = nb.plummer(1500, 6)
+    res = []
+    for graph in (False, True):
+        with nb.Backend("fp64", 1e-4) as b:
+            b.upload(s)
+            b.set_dt(b.initial_dt(1e-4), adaptive=True)
+            b.step(32, graph=graph)
+            res.append((b.download()[0], b.get_dt()))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1]
+
+
+def test_energy_compat_vs_reference_fixture(nb):
+    g = golden("ref_ic_n1024.npz")
+    with nb.Backend("fp64", EPS2_REF) as b:
+        kin, pot = b.energy_of(g["state"], compat=True)
+    assert kin == pytest.approx(float(g["kinetic"]), rel=1e-14)
+    assert pot == pytest.approx(float(g["potential"]), rel=1e-12)
+
+
+def test_energy_physical_vs_oracle(nb, oracle):
+    s = nb.plummer(3000, 13)
+    s[3] *= np.random.default_rng(2).lognormal(0.0, 0.1, 3000)  # unequal masses
+    for eps2 in (1e-4, 0.0):
+        with nb.Backend("fp64", eps2) as b:
+            kin, pot = b.energy_of(s)
+        assert kin == pytest.approx(oracle.kinetic(s), rel=1e-13)
+        assert pot == pytest.approx(oracle.potential(s, eps2, nthreads=16), rel=1e-12)
+
+
+def test_mixed_energy_drift_config2(nb):
+    """N=102400 Plummer seed 2, eps = 1/256, dt = 1/1024, 3 steps: the mixed
+    form's relative energy drift stays within 10x of the FP64 drift (or below
+    1e-10 when the FP64 drift itself is at round-off)."""
+    s = nb.plummer(102400, 2)
+    eps2, dt = (1 / 256) ** 2, 1 / 1024
+    drift = {}
+    for prec in ("fp64", "mixed"):
+        with nb.Backend(prec, eps2) as b:
+            b.upload(s)
+            e0 = sum(b.energy())
+            b.set_dt(dt)
+            b.step(3)
+            drift[prec] = abs(sum(b.energy()) - e0) / abs(e0)
+    _banner(f"config2 energy drift fp64 {drift['fp64']:.3e} mixed {drift['mixed']:.3e}")
+    assert drift["mixed"] <= max(10 * drift["fp64"], 1e-10)
+
+
+def test_step_before_upload_is_state_error(nb):
+    with nb.Backend("fp64", 1e-4) as b:
+        with pytest.raises(nb.NBodyError) as e:
+            b.step(1)
+        assert e.value.code == nb.ERR_STATE
+        b.upload(nb.plummer(64, 1))
+        with pytest.raises(nb.NBodyError) as e:
+            b.step(1)   # no dt yet
+        assert e.value.code == nb.ERR_STATE

@@ -1,0 +1,247 @@
+"""Pins the oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each function is checked against the reference's own outputs (golden
+fixtures written from the compiled reference, and the compiled reference
+itself when oracle/_ref is built here) and against what the mathematics
+fixes: closed forms, conservation laws, exact integration of polynomial
+motion, and long-double brute force.  CPU only.
+"""
+import numpy as np
+import pytest
+
+from conftest import body_relerr, golden
+
+EPS2_REF = 1e-7  # include/kernel.h:13
+
+
+# ---------------------------------------------------------------- fixtures --
+def test_force_f64_bitwise_vs_reference_fixture(oracle):
+    g = golden("ref_ic_n1024.npz")
+    assert np.array_equal(oracle.force(g["state"], EPS2_REF, "fp64"), g["force64"])
+    g = golden("plummer_n1024.npz")
+    assert np.array_equal(oracle.force(g["state"], EPS2_REF, "fp64"), g["force64"])
+
+
+def test_force_f32_bitwise_vs_reference_fixture(oracle):
+    for name in ("ref_ic_n1024.npz", "plummer_n1024.npz"):
+        g = golden(name)
+        assert np.array_equal(oracle.force(g["state"], EPS2_REF, "f32"), g["force32"])
+
+
+def test_energy_bitwise_vs_reference_fixture(oracle):
+    g = golden("ref_ic_n1024.npz")
+    assert oracle.kinetic(g["state"]) == float(g["kinetic"])
+    assert oracle.potential_compat(g["state"], EPS2_REF) == float(g["potential"])
+
+
+def test_reference_ic_bitwise_vs_fixture(oracle):
+    g = golden("ref_ic_n1024.npz")
+    assert np.array_equal(oracle.reference_ic(1024), g["state"])
+
+
+def test_fixed_dt_trajectory_bitwise_vs_fixture(oracle):
+    g = golden("plummer_n1024.npz")
+    out = oracle.run(g["state"], EPS2_REF, float(g["fixed_dt"]), 8)
+    assert np.array_equal(out["state"], g["fixed_state"])
+    assert np.array_equal(out["force"], g["fixed_force"])
+
+
+def test_adaptive_dt_trajectory_bitwise_vs_fixture(oracle):
+    g = golden("plummer_n1024.npz")
+    out = oracle.run(g["state"], EPS2_REF, 0.0, 8, adaptive=True)
+    assert np.array_equal(out["dts"], g["adapt_dts"][:8])
+    assert out["next_dt"] == g["adapt_dts"][8]
+    assert np.array_equal(out["state"], g["adapt_state"])
+    assert np.array_equal(out["force"], g["adapt_force"])
+
+
+@pytest.mark.parametrize("n", [2, 3, 8, 64])
+def test_tiny_systems_bitwise_vs_fixture(oracle, n):
+    g = golden("tiny.npz")
+    out = oracle.run(g[f"state{n}"], EPS2_REF, float(g["dt"]), 4)
+    assert np.array_equal(out["state"], g[f"final{n}"])
+    assert np.array_equal(out["force"], g[f"force{n}"])
+
+
+def test_large_sample_vs_fixture(oracle, nb):
+    """N=102400 reference force (subsampled) vs the long-double directed sum."""
+    import hashlib
+    g = golden("plummer_n102400_sample.npz")
+    s = nb.plummer(int(g["n"]), int(g["seed"]))
+    assert hashlib.sha256(np.ascontiguousarray(s, "<f8").tobytes()).hexdigest() == str(g["ic_sha256"])
+    idx = g["idx"][::8]
+    f = oracle.force_sample(s, EPS2_REF, idx, nthreads=8)
+    ref = g["force64"][:, ::8]
+    assert body_relerr(f[:3], ref[:3]).max() < 1e-12
+    assert body_relerr(f[3:], ref[3:]).max() < 1e-12
+
+
+# ------------------------------------------------- live compiled reference --
+def test_live_reference_bitwise(oracle, reference):
+    r64, r32 = reference
+    xs = np.exp(np.random.default_rng(7).uniform(-40, 40, 20000))
+    assert all(oracle.invsqrt2d(v) == r64.invsqrt2d(v) for v in xs)
+    s = r64.initialize(300)
+    assert np.array_equal(oracle.reference_ic(300), s)
+    assert np.array_equal(oracle.force(s, EPS2_REF), r64.force(s))
+    assert np.array_equal(oracle.force(s, EPS2_REF, "f32"), r32.force(s))
+    out = r64.run(s, 5, 0.0)
+    mine = oracle.run(s, EPS2_REF, 0.0, 5, adaptive=True)
+    assert np.array_equal(mine["state"], out["state"])
+    assert np.array_equal(mine["dts"], out["dts"][:5])
+
+
+# ---------------------------------------------------------- mathematics --
+def test_invsqrt_accuracy(oracle):
+    xs = np.exp(np.random.default_rng(1).uniform(-50, 50, 50000))
+    y = np.array([oracle.invsqrt2d(v) for v in xs])
+    assert np.max(np.abs(y * np.sqrt(xs) - 1.0)) < 5e-16
+
+
+def two_body(m1, m2, d, u, eps2):
+    """Closed-form acc and jerk of body 0 from body 1 at offset d, relative velocity u."""
+    d = np.asarray(d, float)
+    u = np.asarray(u, float)
+    s = d @ d + eps2
+    a = m2 * d / s ** 1.5
+    j = m2 * (u / s ** 1.5 - 3.0 * (d @ u) * d / s ** 2.5)
+    return a, j
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-15), ("f32", 2e-6)])
+def test_two_body_closed_form(oracle, precision, tol):
+    m1, m2 = 0.3, 0.7
+    d = np.array([0.4, -1.1, 0.25])
+    u = np.array([0.05, 0.3, -0.2])
+    eps2 = 1e-3
+    x0 = np.array([0.1, 0.2, -0.3])
+    v0 = np.array([-0.2, 0.1, 0.05])
+    s = np.zeros((7, 2))
+    s[0:3, 0], s[0:3, 1] = x0, x0 + d
+    s[3] = [m1, m2]
+    s[4:7, 0], s[4:7, 1] = v0, v0 + u
+    f = oracle.force(s, eps2, precision)
+    a0, j0 = two_body(m1, m2, d, u, eps2)
+    a1, j1 = two_body(m2, m1, -d, -u, eps2)
+    scale_a, scale_j = np.linalg.norm(a0), np.linalg.norm(j0)
+    assert np.abs(f[0:3, 0] - a0).max() < tol * scale_a
+    assert np.abs(f[3:6, 0] - j0).max() < tol * scale_j
+    assert np.abs(f[0:3, 1] - a1).max() < tol * np.linalg.norm(a1)
+    assert np.abs(f[3:6, 1] - j1).max() < tol * np.linalg.norm(j1)
+
+
+def test_momentum_conservation(oracle, nb):
+    """sum m_i a_i = 0 and sum m_i jerk_i = 0 (Newton's third law)."""
+    s = nb.reference_ic(512)
+    f = oracle.force(s, 1e-4)
+    m = s[3]
+    for c in range(6):
+        scale = np.sum(np.abs(m * f[c]))
+        assert abs(np.sum(m * f[c])) < 1e-13 * scale
+
+
+def test_brute_force_long_double(oracle, nb):
+    s = nb.plummer(64, 9)
+    eps2 = 1.0 / 4096
+    f = oracle.force(s, eps2)
+    ld = oracle.force_sample(s, eps2, np.arange(64))
+    assert body_relerr(f[:3], ld[:3]).max() < 1e-13
+    assert body_relerr(f[3:], ld[3:]).max() < 1e-13
+
+
+def test_translation_and_rotation(oracle, nb):
+    s = nb.plummer(128, 4)
+    eps2 = 1e-3
+    f = oracle.force(s, eps2)
+    t = s.copy()
+    t[0] += 0.375
+    t[4] += 0.25  # a Galilean boost leaves relative velocities unchanged
+    assert body_relerr(oracle.force(t, eps2)[:3], f[:3]).max() < 1e-11
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    r = s.copy()
+    r[0:3] = R @ s[0:3]
+    r[4:7] = R @ s[4:7]
+    fr = oracle.force(r, eps2)
+    assert body_relerr(fr[:3], R @ f[:3]).max() < 1e-12
+    assert body_relerr(fr[3:], R @ f[3:]).max() < 1e-11
+
+
+def test_predictor_exact_for_constant_jerk(oracle):
+    """x(t) = x0 + v0 t + a t^2/2 + j t^3/6 is reproduced by the predictor."""
+    n = 5
+    rng = np.random.default_rng(3)
+    s = np.zeros((7, n))
+    s[0:3] = rng.uniform(-1, 1, (3, n))
+    s[4:7] = rng.uniform(-1, 1, (3, n))
+    f = rng.uniform(-1, 1, (6, n))
+    dt = 0.125  # powers of two keep the closed form exact in binary
+    p = oracle.predict(s, f, dt)
+    xe = s[0:3] + s[4:7] * dt + f[0:3] * dt ** 2 / 2 + f[3:6] * dt ** 3 / 6
+    ve = s[4:7] + f[0:3] * dt + f[3:6] * dt ** 2 / 2
+    assert np.abs(p[0:3] - xe).max() < 1e-15
+    assert np.abs(p[4:7] - ve).max() < 1e-15
+
+
+def test_corrector_recovers_quartic_motion(oracle):
+    """With a(t) a cubic polynomial, snap/crackle from (a0, j0, a1, j1) are the
+    exact 2nd/3rd derivatives at t=0 and the corrected state is the exact
+    Taylor solution through t^5."""
+    n = 3
+    rng = np.random.default_rng(11)
+    a0, j0 = rng.uniform(-1, 1, (3, n)), rng.uniform(-1, 1, (3, n))
+    s2, s3 = rng.uniform(-1, 1, (3, n)), rng.uniform(-1, 1, (3, n))
+    dt = 0.25
+    a1 = a0 + j0 * dt + s2 * dt ** 2 / 2 + s3 * dt ** 3 / 6
+    j1 = j0 + s2 * dt + s3 * dt ** 2 / 2
+    st = np.zeros((7, n))
+    st[0:3] = rng.uniform(-1, 1, (3, n))
+    st[4:7] = rng.uniform(-1, 1, (3, n))
+    f0 = np.vstack([a0, j0])
+    p = oracle.predict(st, f0, dt)
+    c, a2, a3 = oracle.snap_correct(p, f0, np.vstack([a1, j1]), dt)
+    assert np.abs(a2 - s2).max() < 1e-12
+    assert np.abs(a3 - s3).max() < 1e-11
+    x = st[0:3] + st[4:7] * dt + a0 * dt ** 2 / 2 + j0 * dt ** 3 / 6 + s2 * dt ** 4 / 24 + s3 * dt ** 5 / 120
+    v = st[4:7] + a0 * dt + j0 * dt ** 2 / 2 + s2 * dt ** 3 / 6 + s3 * dt ** 4 / 24
+    assert np.abs(c[0:3] - x).max() < 1e-14
+    assert np.abs(c[4:7] - v).max() < 1e-14
+
+
+def test_dt_criteria_hand_values(oracle):
+    f = np.zeros((6, 2))
+    f[0:3, 0] = [3.0, 0.0, 4.0]   # |a| = 5
+    f[3:6, 0] = [0.0, 2.0, 0.0]   # |j| = 2
+    f[0:3, 1] = [1.0, 0.0, 0.0]
+    f[3:6, 1] = [0.0, 0.0, 0.5]   # ratio 2: the minimum
+    assert oracle.initialdt(f, 1e-4) == pytest.approx(1e-4 * 2.0, rel=1e-15)
+    assert oracle.initialdt(f[:, :1], 1e-4) == pytest.approx(1e-4 * 2.5, rel=1e-15)
+    a2 = np.zeros((3, 2)); a3 = np.zeros((3, 2))
+    a2[:, 0] = [0.0, 0.0, 1.0]; a3[:, 0] = [2.0, 0.0, 0.0]
+    a2[:, 1] = [1.0, 0.0, 0.0]; a3[:, 1] = [1.0, 0.0, 0.0]
+    # body 0: num = 5*1 + 4 = 9, den = 2*2 + 1 = 5 -> sqrt(0.01*9/5)
+    # body 1: num = 1*1 + 0.25, den = 0.5*1 + 1 -> sqrt(0.01*1.25/1.5)
+    assert oracle.updatedt(f, a2, a3, 1e-2) == pytest.approx(np.sqrt(0.01 * 1.25 / 1.5), rel=1e-15)
+
+
+def test_potential_closed_form(oracle):
+    s = np.zeros((7, 3))
+    s[0:3, 0] = [0, 0, 0]
+    s[0:3, 1] = [3, 4, 0]
+    s[0:3, 2] = [0, 0, 2]
+    s[3] = [1.0, 2.0, 0.5]
+    eps2 = 0.25
+    u = -(1 * 2 / np.sqrt(25 + eps2) + 1 * 0.5 / np.sqrt(4 + eps2) + 2 * 0.5 / np.sqrt(9 + 16 + 4 + eps2))
+    assert oracle.potential(s, eps2) == pytest.approx(u, rel=1e-15)
+    # the reference's own definition weights pairs by the later body's mass squared
+    uc = -(2 * 2 / np.sqrt(25 + eps2) + 0.5 * 0.5 / np.sqrt(4 + eps2) + 0.5 * 0.5 / np.sqrt(29 + eps2))
+    assert oracle.potential_compat(s, eps2) == pytest.approx(uc, rel=1e-15)
+    s[4:7, 1] = [1.0, 0.0, 2.0]
+    assert oracle.kinetic(s) == 0.5 * 2.0 * 5.0
+
+
+def test_multithreaded_oracle_close_to_serial(oracle, nb):
+    s = nb.plummer(2048, 12)
+    a = oracle.force(s, 1e-4, nthreads=1)
+    b = oracle.force(s, 1e-4, nthreads=4)
+    assert body_relerr(b[:3], a[:3]).max() < 1e-13
