@@ -80,6 +80,39 @@ def run_precision(nb, prec, state, steps, warmup, log):
     return res, f0
 
 
+def run_ensemble(nb, log, nsteps=64):
+    """8192 independent Plummer clusters, n_k log-uniform in [8, 4096]
+    (seed 4), eps_k = 1/(4 n_k^(1/3)), dt = 1/256: autotuned persistent
+    force launch, steps replayed from a hipGraph."""
+    counts = nb.ensemble_sizes(8192, 8, 4096, 4)
+    eps2 = (1.0 / (4.0 * counts.astype(float) ** (1.0 / 3.0))) ** 2
+    state = nb.ensemble_state(counts, 4)
+    pairs = float((counts.astype(np.float64) ** 2).sum())
+    out = {"workload": "8192 Plummer clusters, n_k log-uniform in [8,4096] (seed 4), "
+                       "eps_k=1/(4 n_k^(1/3)), dt=1/256, 64 steps, hipGraph replay",
+           "n_systems": int(len(counts)), "n_bodies": int(counts.sum()), "pairs_per_step": pairs}
+    for prec in ("mixed", "fp64"):
+        with nb.Backend(prec) as b, nb.Ensemble(b, counts, eps2) as e:
+            e.upload(state)
+            tune = e.autotune()
+            e.set_dt(1.0 / 256)
+            e.step(16, graph=True)      # warm-up: builds and replays the graph once
+            k0, u0 = e.energy()
+            t0 = time.perf_counter()
+            e.step(nsteps, graph=True)
+            sec = time.perf_counter() - t0
+            k1, u1 = e.energy()
+            info = e.info()
+        drift = np.abs((k1 + u1) - (k0 + u0)) / np.abs(k0 + u0)
+        out[prec] = {"systems_per_s": len(counts) * nsteps / sec, "pairs_per_s": pairs * nsteps / sec,
+                     "steps_per_s": nsteps / sec, "ms_per_step": 1e3 * sec / nsteps,
+                     "energy_drift_median": float(np.median(drift)), "energy_drift_max": float(drift.max()),
+                     "tuning": tune, "work_items": info["items"], "workgroups": info["workgroups"]}
+        log(f"ensemble {prec}: {1e3 * sec / nsteps:.2f} ms/step, {pairs * nsteps / sec:.3e} pairs/s, "
+            f"{len(counts) * nsteps / sec:.3e} systems/s, drift median {np.median(drift):.1e}, tuning {tune}")
+    return out
+
+
 def cpu_baseline(state, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
@@ -107,6 +140,7 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -126,6 +160,8 @@ def main():
     err = body_relerr(fmx[:3], f64[:3])
     med, mx = float(np.median(err)), float(err.max())
     log(f"mixed vs fp64 acc rel err: median {med:.2e}, max {mx:.2e}")
+
+    ens = None if args.no_ensemble else run_ensemble(nb, log)
 
     cpu = None
     if not args.no_cpu_baseline and rank == 0:
@@ -164,6 +200,7 @@ def main():
                           "kernel": "k_force_f64<256>", "kernel_ms": r64["kernel_ms"],
                           "flop_per_launch": FLOP_PER_PAIR * N * N},
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)

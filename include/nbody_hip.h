@@ -157,6 +157,53 @@ int nbh_launch_shape(nbh_ctx *ctx, int n, int *block, int *jsplit,
 int nbh_device_info(int device, int *compute_units, int *wave_size,
                     char *name, int name_len);
 
+/* ---- batched ensemble of independent systems ------------------------- */
+/* No counterpart in the reference (it integrates one system).  Thousands of
+ * independent clusters of uneven size share one device state: the body
+ * arrays are the systems' arrays concatenated in system order.  The force
+ * evaluation is one persistent launch: one-wave workgroups take work items
+ * (system, block of 64 targets, source range) from a device queue, largest
+ * first; a size bucket's work items use the LDS tile length and j-split
+ * chosen for that bucket (by nbh_ens_autotune or nbh_ens_set_tuning).
+ * Precision and device come from the context.  The step is the shared-dt
+ * Hermite step of nbh_step applied to every system. */
+
+typedef struct nbh_ensemble nbh_ensemble;
+
+/* Size buckets (by bodies per system): n <= 64, <= 256, <= 1024, larger. */
+#define NBH_ENS_BUCKETS 4
+
+/* counts[k] bodies (>= 1) and softening eps2[k] (>= 0) of system k. */
+int nbh_ens_create(nbh_ctx *ctx, int nsys, const int *counts,
+                   const double *eps2, nbh_ensemble **out);
+int nbh_ens_destroy(nbh_ensemble *ens);
+
+/* Copies the concatenated state (sum(counts) bodies per array) to HBM and
+ * evaluates the initial forces. */
+int nbh_ens_upload(nbh_ensemble *ens, const double *x, const double *y,
+                   const double *z, const double *m, const double *vx,
+                   const double *vy, const double *vz);
+int nbh_ens_set_dt(nbh_ensemble *ens, double dt);
+/* nsteps Hermite steps of every system; use_graph != 0 replays a captured
+ * hipGraph of the step sequence.  Synchronous. */
+int nbh_ens_step(nbh_ensemble *ens, int nsteps, int use_graph);
+/* Concatenated state and current forces; NULL skips an array. */
+int nbh_ens_download(nbh_ensemble *ens, double *x, double *y, double *z,
+                     double *m, double *vx, double *vy, double *vz,
+                     double *ax, double *ay, double *az, double *jx,
+                     double *jy, double *jz);
+/* Per-system kinetic and softened potential energy (FP64), arrays [nsys]. */
+int nbh_ens_energy(nbh_ensemble *ens, double *kinetic, double *potential);
+/* Times the candidate (tile in {64,128,256}) x (jsplit in {1,2,4,8}) for
+ * every populated bucket on the device and keeps the fastest.  Outputs
+ * [NBH_ENS_BUCKETS] each, may be NULL; ms = best force time of the bucket
+ * alone (0 for empty buckets). */
+int nbh_ens_autotune(nbh_ensemble *ens, int *tiles, int *jsplits, double *ms);
+/* Sets the per-bucket tile (64, 128 or 256) and jsplit (>= 1) directly. */
+int nbh_ens_set_tuning(nbh_ensemble *ens, const int *tiles, const int *jsplits);
+/* Work items per force evaluation, persistent workgroups, largest tile. */
+int nbh_ens_info(nbh_ensemble *ens, int *items, int *workgroups, int *max_tile);
+
 /* ---- initial conditions (host code, no device needed) --------------- */
 
 /* Equal-mass Plummer sphere in Henon units (G = M = 1, E = -1/4):

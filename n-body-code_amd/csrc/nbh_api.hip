@@ -5,8 +5,7 @@
 // its device buffers across calls, growing them when n grows; the
 // reference's TT dispatch instead opened the device and built buffers on
 // every force_calculation (src/kernel.cpp:54-138, :397).
-#include "nbh_internal.h"
-#include "../../include/nbody_hip.h"
+#include "nbh_ctx.h"
 
 #include <chrono>
 #include <cmath>
@@ -17,9 +16,11 @@
 #include <utility>
 #include <vector>
 
-namespace {
+namespace nbh {
 
+namespace {
 thread_local std::string g_err;
+}
 
 int fail(int code, const char* fmt, ...)
 {
@@ -32,76 +33,63 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define NBH_HIP(call)                                                              \
-    do {                                                                           \
-        hipError_t e_ = (call);                                                    \
-        if (e_ != hipSuccess)                                                      \
-            return fail(NBH_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
+const char* last_error() { return g_err.c_str(); }
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
+State state_of(double* base, int64_t n)
+{
+    State s;
+    s.x = base;
+    s.y = base + n;
+    s.z = base + 2 * n;
+    s.m = base + 3 * n;
+    s.vx = base + 4 * n;
+    s.vy = base + 5 * n;
+    s.vz = base + 6 * n;
+    return s;
+}
+
+Forces forces_of(double* base, int64_t n)
+{
+    Forces f;
+    for (int c = 0; c < 6; ++c) f.c[c] = base + c * n;
+    return f;
+}
+
+int get_events(nbh_ctx* c, EventPair* ep)
+{
+    if (!c->ev_free.empty()) {
+        *ep = c->ev_free.back();
+        c->ev_free.pop_back();
+        return NBH_OK;
     }
-    // Grows to at least `need` bytes (contents not preserved).
-    hipError_t reserve(size_t need)
-    {
-        if (need <= bytes) return hipSuccess;
-        release();
-        hipError_t e = hipMalloc(&p, need);
-        if (e == hipSuccess) bytes = need;
-        return e;
+    NBH_HIP(hipEventCreate(&ep->a));
+    NBH_HIP(hipEventCreate(&ep->b));
+    return NBH_OK;
+}
+
+// Adds the elapsed time of every recorded pair (stream must be idle).
+int settle_events(nbh_ctx* c)
+{
+    for (auto& ep : c->ev_pending) {
+        float ms = 0.f;
+        NBH_HIP(hipEventElapsedTime(&ms, ep.a, ep.b));
+        c->force_ms += ms;
+        c->force_launches += 1;
+        c->ev_free.push_back(ep);
     }
-    double* d() const { return static_cast<double*>(p); }
-};
+    c->ev_pending.clear();
+    return NBH_OK;
+}
 
-struct EventPair {
-    hipEvent_t a = nullptr, b = nullptr;
-};
+}  // namespace nbh
 
-}  // namespace
-
-struct nbh_ctx {
-    nbh_config cfg{};
-    int cus = 0;
-    hipStream_t stream = nullptr;
-
-    // transient buffers of the host-array calls
-    DevBuf in;       // 7n
-    DevBuf out;      // 6n
-    DevBuf partial;  // 6 * jsplit * n
-    DevBuf flag;     // int
-    DevBuf scal;     // scalars: [0] dt, [1] dt_next bits, [2..3] energy, [4..] spare
-
-    // resident state
-    int n = 0;
-    DevBuf state;  // 7n: x y z m vx vy vz
-    DevBuf f0;     // 6n: forces at the current state
-    DevBuf f1;     // 6n: forces at the predicted state
-    DevBuf ework;  // energy block sums
-    bool adaptive = false;
-    double eta = 1e-2;
-    bool dt_set = false;
-
-    // graph of the step sequence
-    hipGraphExec_t gexec = nullptr;
-    int g_steps = 0;
-    int g_n = -1;
-    bool g_adaptive = false;
-
-    // profiling
-    bool profiling = false;
-    std::vector<EventPair> ev_free, ev_pending;
-    double force_ms = 0.0;
-    int64_t force_launches = 0;
-    double step_ms = 0.0;
-    int64_t steps = 0;
-};
+using nbh::DevBuf;
+using nbh::EventPair;
+using nbh::fail;
+using nbh::forces_of;
+using nbh::get_events;
+using nbh::settle_events;
+using nbh::state_of;
 
 namespace {
 
@@ -148,52 +136,6 @@ void drop_graph(nbh_ctx* c)
     c->g_n = -1;
 }
 
-nbh::State state_of(double* base, int n)
-{
-    nbh::State s;
-    s.x = base;
-    s.y = base + (size_t)n;
-    s.z = base + 2 * (size_t)n;
-    s.m = base + 3 * (size_t)n;
-    s.vx = base + 4 * (size_t)n;
-    s.vy = base + 5 * (size_t)n;
-    s.vz = base + 6 * (size_t)n;
-    return s;
-}
-
-nbh::Forces forces_of(double* base, int n)
-{
-    nbh::Forces f;
-    for (int c = 0; c < 6; ++c) f.c[c] = base + (size_t)c * n;
-    return f;
-}
-
-int get_events(nbh_ctx* c, EventPair* ep)
-{
-    if (!c->ev_free.empty()) {
-        *ep = c->ev_free.back();
-        c->ev_free.pop_back();
-        return NBH_OK;
-    }
-    NBH_HIP(hipEventCreate(&ep->a));
-    NBH_HIP(hipEventCreate(&ep->b));
-    return NBH_OK;
-}
-
-// Adds the elapsed time of every recorded pair (stream must be idle).
-int settle_events(nbh_ctx* c)
-{
-    for (auto& ep : c->ev_pending) {
-        float ms = 0.f;
-        NBH_HIP(hipEventElapsedTime(&ms, ep.a, ep.b));
-        c->force_ms += ms;
-        c->force_launches += 1;
-        c->ev_free.push_back(ep);
-    }
-    c->ev_pending.clear();
-    return NBH_OK;
-}
-
 // One force evaluation on device arrays, enqueued on st.
 int enqueue_force(nbh_ctx* c, const nbh::State& s, int n, const nbh::Forces& out,
                   hipStream_t st, bool allow_events)
@@ -223,7 +165,7 @@ bool bad_n(int n) { return n < 1; }
 
 extern "C" {
 
-const char* nbh_last_error(void) { return g_err.c_str(); }
+const char* nbh_last_error(void) { return nbh::last_error(); }
 
 int nbh_version(void) { return 100; }
 

@@ -1,0 +1,680 @@
+// nbh_ensemble.hip -- many independent systems advanced together.
+//
+// Layout: the systems' SoA arrays are concatenated in system order (system k
+// occupies [off_k, off_k + n_k)), so the predictor and corrector kernels of
+// the single-system path run unchanged over all bodies at once.
+//
+// Force evaluation: one persistent launch of one-wave (64-lane) workgroups.
+// Each wave takes work items from a device queue with an atomic counter:
+// item = (system, 64 consecutive targets, a source range, an LDS tile
+// length, an output slot).  Items are sorted by cost, largest first, so the
+// long items start early and the short ones fill the tail.  Systems whose
+// bucket uses a j-split write per-item partial sums into slots that a second
+// kernel adds in split order (deterministic).  Every wave leaves the loop
+// once the counter passes the item count, so the grid always drains.
+#include "nbh_ctx.h"
+#include "nbh_pair.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <type_traits>
+#include <vector>
+
+namespace nbh {
+namespace {
+
+constexpr int kLanes = 64;
+const int kBucketEdge[NBH_ENS_BUCKETS] = {64, 256, 1024, INT_MAX};
+const int kTiles[3] = {64, 128, 256};
+const int kSplits[4] = {1, 2, 4, 8};
+
+struct EnsSys {
+    int off;
+    int n;
+    double eps2;
+};
+
+struct EnsItem {
+    int sys;
+    int i0;      // first target (global index)
+    int icount;  // targets in this item (<= 64)
+    int j0, j1;  // source range (global indices)
+    int tile;    // LDS tile length for this item
+    int slot;    // partial-sum slot, or -1: write the forces directly
+    int pad;
+};
+
+struct EnsReduce {
+    int i0;
+    int icount;
+    int slot0;
+    int splits;
+};
+
+struct EnsArgs {
+    State s;
+    Forces out;
+    const EnsSys* sys;
+    const EnsItem* items;
+    int nitems;
+    unsigned* counter;
+    double* partial;  // [slot][6][64]
+};
+
+int bucket_of(int n)
+{
+    for (int b = 0; b < NBH_ENS_BUCKETS; ++b)
+        if (n <= kBucketEdge[b]) return b;
+    return NBH_ENS_BUCKETS - 1;
+}
+
+// Lane 0 takes the next item; the index is made wave-uniform (an SGPR), so
+// the item and its loop bounds are scalar loads and scalar branches.
+__device__ __forceinline__ int take_item(unsigned* counter)
+{
+    int w = 0;
+    if (threadIdx.x == 0) w = (int)atomicAdd(counter, 1u);
+    return __builtin_amdgcn_readfirstlane(w);
+}
+
+template <bool MIXED, int MAXT, bool MASK>
+__global__ __launch_bounds__(kLanes) void k_ens_force(EnsArgs a)
+{
+    using JB = typename std::conditional<MIXED, JBMX, JB64>::type;
+    __shared__ JB tile[MAXT];
+    const int lane = threadIdx.x;
+    for (;;) {
+        const int w = take_item(a.counter);
+        if (w >= a.nitems) break;
+        const EnsItem it = a.items[w];
+        const double eps2 = a.sys[it.sys].eps2;
+        const int i = it.i0 + lane;
+        const int ic = lane < it.icount ? i : it.i0;
+        const double xi = a.s.x[ic], yi = a.s.y[ic], zi = a.s.z[ic];
+        double out[6];
+        if constexpr (MIXED) {
+            const float vxi = (float)a.s.vx[ic], vyi = (float)a.s.vy[ic], vzi = (float)a.s.vz[ic];
+            const float e2 = (float)eps2;
+            float hi[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, lo[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int jt = it.j0; jt < it.j1; jt += it.tile) {
+                const int cnt = min(it.tile, it.j1 - jt);
+                __syncthreads();
+                for (int k = lane; k < cnt; k += kLanes) tile[k] = load_jbmx(a.s, jt + k);
+                __syncthreads();
+                for (int k0 = 0; k0 < cnt; k0 += kFold) {
+                    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    const int kend = min(cnt, k0 + kFold);
+                    if (kend - k0 == kFold) {
+#pragma unroll 4
+                        for (int k = k0; k < k0 + kFold; ++k)
+                            pair_mx<MASK>(tile[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, e2, acc);
+                    } else {
+                        for (int k = k0; k < kend; ++k)
+                            pair_mx<MASK>(tile[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, e2, acc);
+                    }
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) two_sum_fold(hi[c], lo[c], acc[c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] = (double)hi[c] + (double)lo[c];
+        } else {
+            const double vxi = a.s.vx[ic], vyi = a.s.vy[ic], vzi = a.s.vz[ic];
+            double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int jt = it.j0; jt < it.j1; jt += it.tile) {
+                const int cnt = min(it.tile, it.j1 - jt);
+                __syncthreads();
+                for (int k = lane; k < cnt; k += kLanes) tile[k] = load_jb64(a.s, jt + k);
+                __syncthreads();
+                if (cnt == it.tile) {
+#pragma unroll 4
+                    for (int k = 0; k < cnt; ++k)
+                        pair_f64<MASK>(tile[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+                } else {
+                    for (int k = 0; k < cnt; ++k)
+                        pair_f64<MASK>(tile[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] = acc[c];
+        }
+        if (lane < it.icount) {
+            if (it.slot < 0) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) a.out.c[c][i] = out[c];
+            } else {
+                double* p = a.partial + (size_t)it.slot * 6 * kLanes + lane;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) p[c * kLanes] = out[c];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kLanes) void k_ens_reduce(const EnsReduce* r, int nr,
+                                                     const double* __restrict__ partial, Forces out)
+{
+    const int b = blockIdx.x;
+    if (b >= nr) return;
+    const EnsReduce q = r[b];
+    const int lane = threadIdx.x;
+    if (lane >= q.icount) return;
+    for (int c = 0; c < 6; ++c) {
+        double v = 0.0;
+        for (int s = 0; s < q.splits; ++s) v += partial[((size_t)(q.slot0 + s) * 6 + c) * kLanes + lane];
+        out.c[c][q.i0 + lane] = v;
+    }
+}
+
+__global__ void k_zero_u32(unsigned* p)
+{
+    if (threadIdx.x == 0) *p = 0u;
+}
+
+// Per-system kinetic and softened potential energy (FP64), one workgroup per
+// system, block sums in a fixed order.
+constexpr int kEnBlock = 256;
+__global__ __launch_bounds__(kEnBlock) void k_ens_energy(State s, const EnsSys* sys, double* kin,
+                                                         double* pot)
+{
+    __shared__ double rk[kEnBlock], rp[kEnBlock];
+    const EnsSys q = sys[blockIdx.x];
+    const int t = threadIdx.x;
+    double k = 0.0, p = 0.0;
+    for (int i = q.off + t; i < q.off + q.n; i += kEnBlock) {
+        const double xi = s.x[i], yi = s.y[i], zi = s.z[i];
+        double phi = 0.0;
+        for (int j = q.off; j < q.off + q.n; ++j) {
+            const double dx = s.x[j] - xi, dy = s.y[j] - yi, dz = s.z[j] - zi;
+            double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, q.eps2)));
+            const bool keep = j != i;
+            r2 = keep ? r2 : 1.0;
+            phi = fma(keep ? s.m[j] : 0.0, rsqrt_f64(r2), phi);
+        }
+        const double mi = s.m[i];
+        p += -0.5 * mi * phi;
+        k += 0.5 * mi * (s.vx[i] * s.vx[i] + s.vy[i] * s.vy[i] + s.vz[i] * s.vz[i]);
+    }
+    rk[t] = k;
+    rp[t] = p;
+    __syncthreads();
+    for (int h = kEnBlock / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            rk[t] += rk[t + h];
+            rp[t] += rp[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        kin[blockIdx.x] = rk[0];
+        pot[blockIdx.x] = rp[0];
+    }
+}
+
+template <bool MIXED, int MAXT, bool MASK>
+const void* ens_kernel_ptr()
+{
+    return reinterpret_cast<const void*>(&k_ens_force<MIXED, MAXT, MASK>);
+}
+
+const void* pick_ens_kernel(bool mixed, int maxt, bool mask)
+{
+#define NBH_ENS_K(M, T)                                                                  \
+    if (mixed == M && maxt == T) return mask ? ens_kernel_ptr<M, T, true>() : ens_kernel_ptr<M, T, false>();
+    NBH_ENS_K(false, 64)
+    NBH_ENS_K(false, 128)
+    NBH_ENS_K(false, 256)
+    NBH_ENS_K(true, 64)
+    NBH_ENS_K(true, 128)
+    NBH_ENS_K(true, 256)
+#undef NBH_ENS_K
+    return nullptr;
+}
+
+}  // namespace
+}  // namespace nbh
+
+using nbh::DevBuf;
+using nbh::fail;
+
+struct nbh_ensemble {
+    nbh_ctx* ctx = nullptr;
+    int nsys = 0;
+    int total = 0;
+    std::vector<int> off, cnt;
+    std::vector<double> eps2;
+    bool any_zero_eps = false;
+    int tile[NBH_ENS_BUCKETS] = {128, 128, 128, 128};
+    int split[NBH_ENS_BUCKETS] = {1, 1, 1, 1};
+
+    // device
+    DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot;
+    int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0;
+    bool uploaded = false, dt_set = false;
+    hipGraphExec_t gexec = nullptr;
+    int g_steps = 0;
+};
+
+namespace {
+
+using nbh::EnsItem;
+using nbh::EnsReduce;
+
+void drop_graph(nbh_ensemble* e)
+{
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    e->gexec = nullptr;
+    e->g_steps = 0;
+}
+
+// Work list for the systems selected by `use` (all when empty), with the
+// per-bucket tile/split of `tile`/`split`.
+void build_items(const nbh_ensemble* e, const int* tile, const int* split,
+                 const std::vector<char>& use, std::vector<EnsItem>& items,
+                 std::vector<EnsReduce>& reds, int& nslots)
+{
+    items.clear();
+    reds.clear();
+    nslots = 0;
+    for (int k = 0; k < e->nsys; ++k) {
+        if (!use.empty() && !use[k]) continue;
+        const int n = e->cnt[k], off = e->off[k];
+        const int b = nbh::bucket_of(n);
+        const int T = tile[b];
+        int S = std::min(split[b], (n + T - 1) / T);
+        S = std::max(S, 1);
+        const int chunk = (n + S - 1) / S;
+        S = (n + chunk - 1) / chunk;
+        for (int ib = 0; ib < n; ib += nbh::kLanes) {
+            const int ic = std::min(nbh::kLanes, n - ib);
+            if (S == 1) {
+                items.push_back({k, off + ib, ic, off, off + n, T, -1, 0});
+            } else {
+                reds.push_back({off + ib, ic, nslots, S});
+                for (int s = 0; s < S; ++s)
+                    items.push_back({k, off + ib, ic, off + s * chunk, off + std::min(n, (s + 1) * chunk), T,
+                                     nslots + s, 0});
+                nslots += S;
+            }
+        }
+    }
+    // largest source range first; stable keeps system order among equals
+    std::stable_sort(items.begin(), items.end(),
+                     [](const EnsItem& a, const EnsItem& b) { return (a.j1 - a.j0) > (b.j1 - b.j0); });
+}
+
+int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::vector<EnsReduce>& reds,
+                 int nslots, int maxtile)
+{
+    nbh_ctx* c = e->ctx;
+    if (e->items.reserve(sizeof(EnsItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
+        e->reduces.reserve(sizeof(EnsReduce) * std::max<size_t>(reds.size(), 1)) != hipSuccess ||
+        e->partial.reserve(sizeof(double) * 6 * nbh::kLanes * std::max(nslots, 1)) != hipSuccess)
+        return fail(NBH_ERR_ALLOC, "ensemble work-list allocation failed");
+    if (!items.empty())
+        NBH_HIP(hipMemcpyAsync(e->items.p, items.data(), sizeof(EnsItem) * items.size(), hipMemcpyHostToDevice,
+                               c->stream));
+    if (!reds.empty())
+        NBH_HIP(hipMemcpyAsync(e->reduces.p, reds.data(), sizeof(EnsReduce) * reds.size(),
+                               hipMemcpyHostToDevice, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    e->nitems = (int)items.size();
+    e->nreduce = (int)reds.size();
+    e->nslots = nslots;
+    e->maxtile = maxtile;
+    const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, maxtile, e->any_zero_eps);
+    int per_cu = 0;
+    NBH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, nbh::kLanes, 0));
+    e->grid = std::max(1, std::min(e->nitems, per_cu * c->cus));
+    return NBH_OK;
+}
+
+int rebuild(nbh_ensemble* e)
+{
+    std::vector<EnsItem> items;
+    std::vector<EnsReduce> reds;
+    int nslots = 0;
+    build_items(e, e->tile, e->split, {}, items, reds, nslots);
+    int maxt = 64;
+    for (const auto& it : items) maxt = std::max(maxt, it.tile);
+    drop_graph(e);
+    return upload_items(e, items, reds, nslots, maxt);
+}
+
+// Force evaluation of the current work list: state -> out.
+int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events)
+{
+    nbh_ctx* c = e->ctx;
+    nbh::EnsArgs a;
+    a.s = nbh::state_of(e->state.d(), e->total);
+    a.out = out;
+    a.sys = static_cast<const nbh::EnsSys*>(e->sys.p);
+    a.items = static_cast<const EnsItem*>(e->items.p);
+    a.nitems = e->nitems;
+    a.counter = static_cast<unsigned*>(e->counter.p);
+    a.partial = e->partial.d();
+    nbh::k_zero_u32<<<1, 64, 0, c->stream>>>(a.counter);
+    NBH_HIP(hipGetLastError());
+    nbh::EventPair ep;
+    const bool timed = c->profiling && allow_events;
+    if (timed) {
+        int rc = nbh::get_events(c, &ep);
+        if (rc) return rc;
+        NBH_HIP(hipEventRecord(ep.a, c->stream));
+    }
+    const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, e->maxtile, e->any_zero_eps);
+    if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d", e->maxtile);
+    void* args[] = {&a};
+    NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, c->stream));
+    if (timed) {
+        NBH_HIP(hipEventRecord(ep.b, c->stream));
+        c->ev_pending.push_back(ep);
+    }
+    if (e->nreduce > 0) {
+        nbh::k_ens_reduce<<<e->nreduce, nbh::kLanes, 0, c->stream>>>(static_cast<const EnsReduce*>(e->reduces.p),
+                                                                      e->nreduce, e->partial.d(), out);
+        NBH_HIP(hipGetLastError());
+    }
+    return NBH_OK;
+}
+
+int enqueue_ens_step(nbh_ensemble* e, bool allow_events)
+{
+    nbh_ctx* c = e->ctx;
+    const int n = e->total;
+    double* b = e->state.d();
+    const nbh::Forces f0 = nbh::forces_of(e->f0.d(), n);
+    const nbh::Forces f1 = nbh::forces_of(e->f1.d(), n);
+    double* dt = e->dt.d();
+    double *x = b, *y = b + n, *z = b + 2 * (size_t)n;
+    double *vx = b + 4 * (size_t)n, *vy = b + 5 * (size_t)n, *vz = b + 6 * (size_t)n;
+    NBH_HIP(nbh::launch_predict(n, dt, x, y, z, vx, vy, vz, f0, c->stream));
+    int rc = enqueue_ens_force(e, f1, allow_events);
+    if (rc) return rc;
+    NBH_HIP(nbh::launch_correct(n, dt, x, y, z, vx, vy, vz, f0, f1, nullptr, 0.0, c->stream));
+    return NBH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbh_ens_create(nbh_ctx* ctx, int nsys, const int* counts, const double* eps2, nbh_ensemble** out)
+{
+    if (!out) return fail(NBH_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!ctx) return fail(NBH_ERR_ARG, "context is NULL");
+    if (nsys < 1 || !counts || !eps2) return fail(NBH_ERR_ARG, "need nsys >= 1 and counts/eps2 arrays");
+    long long total = 0;
+    for (int k = 0; k < nsys; ++k) {
+        if (counts[k] < 1) return fail(NBH_ERR_ARG, "system %d has %d bodies", k, counts[k]);
+        if (!(eps2[k] >= 0.0) || !std::isfinite(eps2[k]))
+            return fail(NBH_ERR_ARG, "system %d: eps2 must be finite and >= 0", k);
+        total += counts[k];
+    }
+    if (total > INT_MAX / 8) return fail(NBH_ERR_ARG, "ensemble too large (%lld bodies)", total);
+    NBH_HIP(hipSetDevice(ctx->cfg.device));
+    auto* e = new (std::nothrow) nbh_ensemble();
+    if (!e) return fail(NBH_ERR_ALLOC, "ensemble allocation failed");
+    e->ctx = ctx;
+    e->nsys = nsys;
+    e->total = (int)total;
+    e->cnt.assign(counts, counts + nsys);
+    e->eps2.assign(eps2, eps2 + nsys);
+    e->off.resize(nsys);
+    std::vector<nbh::EnsSys> sys(nsys);
+    int o = 0;
+    for (int k = 0; k < nsys; ++k) {
+        e->off[k] = o;
+        sys[k] = {o, counts[k], eps2[k]};
+        o += counts[k];
+        if (eps2[k] == 0.0) e->any_zero_eps = true;
+    }
+    const size_t nb = sizeof(double) * (size_t)total;
+    if (e->sys.reserve(sizeof(nbh::EnsSys) * nsys) != hipSuccess || e->state.reserve(7 * nb) != hipSuccess ||
+        e->f0.reserve(6 * nb) != hipSuccess || e->f1.reserve(6 * nb) != hipSuccess ||
+        e->counter.reserve(sizeof(unsigned)) != hipSuccess || e->dt.reserve(sizeof(double)) != hipSuccess ||
+        e->ekin.reserve(sizeof(double) * nsys) != hipSuccess || e->epot.reserve(sizeof(double) * nsys) != hipSuccess) {
+        nbh_ens_destroy(e);
+        return fail(NBH_ERR_ALLOC, "ensemble device allocation failed (%lld bodies)", total);
+    }
+    hipError_t he = hipMemcpy(e->sys.p, sys.data(), sizeof(nbh::EnsSys) * nsys, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        nbh_ens_destroy(e);
+        return fail(NBH_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(he));
+    }
+    int rc = rebuild(e);
+    if (rc) {
+        nbh_ens_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return NBH_OK;
+}
+
+int nbh_ens_destroy(nbh_ensemble* e)
+{
+    if (!e) return NBH_OK;
+    (void)hipSetDevice(e->ctx->cfg.device);
+    (void)hipStreamSynchronize(e->ctx->stream);
+    drop_graph(e);
+    for (DevBuf* b : {&e->sys, &e->state, &e->f0, &e->f1, &e->items, &e->reduces, &e->partial, &e->counter,
+                      &e->dt, &e->ekin, &e->epot})
+        b->release();
+    delete e;
+    return NBH_OK;
+}
+
+int nbh_ens_upload(nbh_ensemble* e, const double* x, const double* y, const double* z, const double* m,
+                   const double* vx, const double* vy, const double* vz)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    const double* ins[7] = {x, y, z, m, vx, vy, vz};
+    for (auto p : ins)
+        if (!p) return fail(NBH_ERR_ARG, "input array is NULL");
+    nbh_ctx* c = e->ctx;
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const size_t nb = sizeof(double) * (size_t)e->total;
+    for (int k = 0; k < 7; ++k)
+        NBH_HIP(hipMemcpyAsync(e->state.d() + (size_t)k * e->total, ins[k], nb, hipMemcpyHostToDevice, c->stream));
+    int rc = enqueue_ens_force(e, nbh::forces_of(e->f0.d(), e->total), false);
+    if (rc) return rc;
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    e->uploaded = true;
+    return NBH_OK;
+}
+
+int nbh_ens_set_dt(nbh_ensemble* e, double dt)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(NBH_ERR_ARG, "dt must be finite and > 0, got %g", dt);
+    NBH_HIP(hipSetDevice(e->ctx->cfg.device));
+    NBH_HIP(hipMemcpy(e->dt.p, &dt, sizeof dt, hipMemcpyHostToDevice));
+    e->dt_set = true;
+    return NBH_OK;
+}
+
+int nbh_ens_step(nbh_ensemble* e, int nsteps, int use_graph)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (nsteps < 0) return fail(NBH_ERR_ARG, "nsteps must be >= 0, got %d", nsteps);
+    if (!e->uploaded) return fail(NBH_ERR_STATE, "no state: call nbh_ens_upload first");
+    if (!e->dt_set) return fail(NBH_ERR_STATE, "no time step: call nbh_ens_set_dt first");
+    nbh_ctx* c = e->ctx;
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const auto t0 = std::chrono::steady_clock::now();
+    int done = 0;
+    if (use_graph && nsteps > 0) {
+        const int chunk = std::min(nsteps, 16);
+        if (!e->gexec || e->g_steps != chunk) {
+            drop_graph(e);
+            NBH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = NBH_OK;
+            for (int k = 0; k < chunk && rc == NBH_OK; ++k) rc = enqueue_ens_step(e, false);
+            hipGraph_t g = nullptr;
+            hipError_t he = hipStreamEndCapture(c->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (he != hipSuccess) return fail(NBH_ERR_HIP, "graph capture failed: %s", hipGetErrorString(he));
+            he = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (he != hipSuccess) {
+                e->gexec = nullptr;
+                return fail(NBH_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(he));
+            }
+            e->g_steps = chunk;
+        }
+        while (nsteps - done >= e->g_steps) {
+            NBH_HIP(hipGraphLaunch(e->gexec, c->stream));
+            done += e->g_steps;
+        }
+    }
+    for (; done < nsteps; ++done) {
+        int rc = enqueue_ens_step(e, true);
+        if (rc) return rc;
+    }
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    c->step_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    c->steps += nsteps;
+    return nbh::settle_events(c);
+}
+
+int nbh_ens_download(nbh_ensemble* e, double* x, double* y, double* z, double* m, double* vx, double* vy,
+                     double* vz, double* ax, double* ay, double* az, double* jx, double* jy, double* jz)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (!e->uploaded) return fail(NBH_ERR_STATE, "no state: call nbh_ens_upload first");
+    nbh_ctx* c = e->ctx;
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const size_t nb = sizeof(double) * (size_t)e->total;
+    double* so[7] = {x, y, z, m, vx, vy, vz};
+    double* fo[6] = {ax, ay, az, jx, jy, jz};
+    for (int k = 0; k < 7; ++k)
+        if (so[k])
+            NBH_HIP(hipMemcpyAsync(so[k], e->state.d() + (size_t)k * e->total, nb, hipMemcpyDeviceToHost, c->stream));
+    for (int k = 0; k < 6; ++k)
+        if (fo[k])
+            NBH_HIP(hipMemcpyAsync(fo[k], e->f0.d() + (size_t)k * e->total, nb, hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    return NBH_OK;
+}
+
+int nbh_ens_energy(nbh_ensemble* e, double* kin, double* pot)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (!kin || !pot) return fail(NBH_ERR_ARG, "output arrays are NULL");
+    if (!e->uploaded) return fail(NBH_ERR_STATE, "no state: call nbh_ens_upload first");
+    nbh_ctx* c = e->ctx;
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    nbh::k_ens_energy<<<e->nsys, nbh::kEnBlock, 0, c->stream>>>(nbh::state_of(e->state.d(), e->total),
+                                                                static_cast<const nbh::EnsSys*>(e->sys.p),
+                                                                e->ekin.d(), e->epot.d());
+    NBH_HIP(hipGetLastError());
+    NBH_HIP(hipMemcpyAsync(kin, e->ekin.p, sizeof(double) * e->nsys, hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipMemcpyAsync(pot, e->epot.p, sizeof(double) * e->nsys, hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    return NBH_OK;
+}
+
+int nbh_ens_set_tuning(nbh_ensemble* e, const int* tiles, const int* jsplits)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (!tiles || !jsplits) return fail(NBH_ERR_ARG, "tiles/jsplits are NULL");
+    for (int b = 0; b < NBH_ENS_BUCKETS; ++b) {
+        if (tiles[b] != 64 && tiles[b] != 128 && tiles[b] != 256)
+            return fail(NBH_ERR_ARG, "bucket %d: tile must be 64, 128 or 256, got %d", b, tiles[b]);
+        if (jsplits[b] < 1) return fail(NBH_ERR_ARG, "bucket %d: jsplit must be >= 1, got %d", b, jsplits[b]);
+    }
+    NBH_HIP(hipSetDevice(e->ctx->cfg.device));
+    std::copy(tiles, tiles + NBH_ENS_BUCKETS, e->tile);
+    std::copy(jsplits, jsplits + NBH_ENS_BUCKETS, e->split);
+    return rebuild(e);
+}
+
+int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (!e->uploaded) return fail(NBH_ERR_STATE, "no state: call nbh_ens_upload first");
+    nbh_ctx* c = e->ctx;
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    hipEvent_t a, b;
+    NBH_HIP(hipEventCreate(&a));
+    NBH_HIP(hipEventCreate(&b));
+    double best_ms[NBH_ENS_BUCKETS] = {0, 0, 0, 0};
+    int rc = NBH_OK;
+    const bool prof = c->profiling;
+    c->profiling = false;
+    for (int bk = 0; bk < NBH_ENS_BUCKETS && rc == NBH_OK; ++bk) {
+        std::vector<char> use(e->nsys, 0);
+        bool any = false;
+        for (int k = 0; k < e->nsys; ++k)
+            if (nbh::bucket_of(e->cnt[k]) == bk) use[k] = 1, any = true;
+        if (!any) continue;
+        double best = 1e300;
+        int bt = e->tile[bk], bs = e->split[bk];
+        for (int T : nbh::kTiles) {
+            for (int S : nbh::kSplits) {
+                int tl[NBH_ENS_BUCKETS], sp[NBH_ENS_BUCKETS];
+                std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tl);
+                std::copy(e->split, e->split + NBH_ENS_BUCKETS, sp);
+                tl[bk] = T;
+                sp[bk] = S;
+                std::vector<EnsItem> items;
+                std::vector<EnsReduce> reds;
+                int nslots = 0;
+                build_items(e, tl, sp, use, items, reds, nslots);
+                if ((rc = upload_items(e, items, reds, nslots, T))) break;
+                float t = 0.f;
+                double mn = 1e300;
+                for (int rep = 0; rep < 3 && rc == NBH_OK; ++rep) {
+                    if (hipEventRecord(a, c->stream) != hipSuccess) rc = fail(NBH_ERR_HIP, "event record failed");
+                    if (rc == NBH_OK) rc = enqueue_ens_force(e, nbh::forces_of(e->f1.d(), e->total), false);
+                    if (rc == NBH_OK && hipEventRecord(b, c->stream) != hipSuccess)
+                        rc = fail(NBH_ERR_HIP, "event record failed");
+                    if (rc == NBH_OK && hipEventSynchronize(b) != hipSuccess) rc = fail(NBH_ERR_HIP, "sync failed");
+                    if (rc == NBH_OK && hipEventElapsedTime(&t, a, b) != hipSuccess)
+                        rc = fail(NBH_ERR_HIP, "elapsed time failed");
+                    if (rep > 0) mn = std::min(mn, (double)t);  // first run warms up
+                }
+                if (rc) break;
+                if (mn < best) {
+                    best = mn;
+                    bt = T;
+                    bs = S;
+                }
+            }
+            if (rc) break;
+        }
+        e->tile[bk] = bt;
+        e->split[bk] = bs;
+        best_ms[bk] = best;
+    }
+    c->profiling = prof;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    int rc2 = rebuild(e);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (tiles) std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tiles);
+    if (jsplits) std::copy(e->split, e->split + NBH_ENS_BUCKETS, jsplits);
+    if (ms_out) std::copy(best_ms, best_ms + NBH_ENS_BUCKETS, ms_out);
+    return NBH_OK;
+}
+
+int nbh_ens_info(nbh_ensemble* e, int* items, int* workgroups, int* max_tile)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (items) *items = e->nitems;
+    if (workgroups) *workgroups = e->grid;
+    if (max_tile) *max_tile = e->maxtile;
+    return NBH_OK;
+}
+
+}  // extern "C"

@@ -21,6 +21,7 @@
 // which is the reference's per-pair formula (src/kernel.cpp:1188-1205)
 // summed over all j != i instead of over the j < i triangle.
 #include "nbh_internal.h"
+#include "nbh_pair.h"
 
 #include <cmath>
 
@@ -28,29 +29,6 @@ namespace nbh {
 namespace {
 
 constexpr int kOBlock = 256;  // threads per workgroup of the O(n) kernels
-
-// FP64 source body in LDS: 64 bytes, read as three 16-byte and one 8-byte
-// broadcast (pad is never read).
-struct __align__(16) JB64 {
-    double x, y, z, m, vx, vy, vz, pad;
-};
-
-// Mixed-precision source body in LDS: FP32 velocity and mass first, FP64
-// position after, 48 bytes -> two 16-byte and one 8-byte broadcast.
-struct __align__(16) JBMX {
-    float vx, vy, vz, m;
-    double x, y, z, pad;
-};
-
-// s^-1/2 in FP64: v_rsq_f64 seed refined by one third-order step,
-// y' = y + y e (1/2 + 3/8 e) with e = 1 - s y^2.  The seed's relative error
-// d becomes O(d^3) plus the rounding of e, i.e. about one ulp.
-__device__ __forceinline__ double rsqrt_f64(double s)
-{
-    const double y = __builtin_amdgcn_rsq(s);
-    const double e = fma(-s, y * y, 1.0);
-    return fma(y * e, fma(e, 0.375, 0.5), y);
-}
 
 struct PairArgs {
     State s;
@@ -60,67 +38,6 @@ struct PairArgs {
     Forces out;
     size_t split_stride;  // element offset between j-split partial planes
 };
-
-__device__ __forceinline__ JB64 load_jb64(const State& s, int j)
-{
-    JB64 b;
-    b.x = s.x[j];
-    b.y = s.y[j];
-    b.z = s.z[j];
-    b.m = s.m[j];
-    b.vx = s.vx[j];
-    b.vy = s.vy[j];
-    b.vz = s.vz[j];
-    b.pad = 0.0;
-    return b;
-}
-
-__device__ __forceinline__ JBMX load_jbmx(const State& s, int j)
-{
-    JBMX b;
-    b.vx = (float)s.vx[j];
-    b.vy = (float)s.vy[j];
-    b.vz = (float)s.vz[j];
-    b.m = (float)s.m[j];
-    b.x = s.x[j];
-    b.y = s.y[j];
-    b.z = s.z[j];
-    b.pad = 0.0;
-    return b;
-}
-
-// ---------------------------------------------------------------- FP64 --
-template <bool MASK>
-__device__ __forceinline__ void pair_f64(const JB64& b, int j, int i,
-                                         double xi, double yi, double zi,
-                                         double vxi, double vyi, double vzi,
-                                         double eps2, double (&acc)[6])
-{
-    const double dx = b.x - xi;
-    const double dy = b.y - yi;
-    const double dz = b.z - zi;
-    const double ux = b.vx - vxi;
-    const double uy = b.vy - vyi;
-    const double uz = b.vz - vzi;
-    double s = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
-    const double vr = fma(dx, ux, fma(dy, uy, dz * uz));
-    double mj = b.m;
-    if (MASK) {  // eps2 == 0: drop the self pair instead of dividing by 0
-        const bool self = (j == i);
-        s = self ? 1.0 : s;
-        mj = self ? 0.0 : mj;
-    }
-    const double r = rsqrt_f64(s);
-    const double r2 = r * r;
-    const double mr3 = mj * r * r2;
-    const double al = -3.0 * vr * r2;
-    acc[0] = fma(mr3, dx, acc[0]);
-    acc[1] = fma(mr3, dy, acc[1]);
-    acc[2] = fma(mr3, dz, acc[2]);
-    acc[3] = fma(mr3, fma(al, dx, ux), acc[3]);
-    acc[4] = fma(mr3, fma(al, dy, uy), acc[4]);
-    acc[5] = fma(mr3, fma(al, dz, uz), acc[5]);
-}
 
 template <int BLOCK, bool MASK>
 __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
@@ -163,54 +80,6 @@ __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
 #pragma unroll
         for (int c = 0; c < 6; ++c) a.out.c[c][o] = acc[c];
     }
-}
-
-// --------------------------------------------------------------- mixed --
-// Position differences are formed in FP64 and rounded; everything after is
-// FP32.  Terms are summed in FP32 over runs of kFold sources, and each run
-// total is folded into an FP32 (hi, lo) pair with Knuth's two-sum, so the
-// rounding error of the long sum is carried in lo.
-constexpr int kFold = 64;
-
-__device__ __forceinline__ void two_sum_fold(float& hi, float& lo, float v)
-{
-    const float s = hi + v;
-    const float bv = s - hi;
-    const float err = (hi - (s - bv)) + (v - bv);
-    hi = s;
-    lo += err;
-}
-
-template <bool MASK>
-__device__ __forceinline__ void pair_mx(const JBMX& b, int j, int i,
-                                        double xi, double yi, double zi,
-                                        float vxi, float vyi, float vzi,
-                                        float eps2, float (&acc)[6])
-{
-    const float dx = (float)(b.x - xi);
-    const float dy = (float)(b.y - yi);
-    const float dz = (float)(b.z - zi);
-    const float ux = b.vx - vxi;
-    const float uy = b.vy - vyi;
-    const float uz = b.vz - vzi;
-    float s = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-    const float vr = fmaf(dx, ux, fmaf(dy, uy, dz * uz));
-    float mj = b.m;
-    if (MASK) {
-        const bool self = (j == i);
-        s = self ? 1.0f : s;
-        mj = self ? 0.0f : mj;
-    }
-    const float r = __builtin_amdgcn_rsqf(s);
-    const float r2 = r * r;
-    const float mr3 = mj * r * r2;
-    const float al = -3.0f * vr * r2;
-    acc[0] = fmaf(mr3, dx, acc[0]);
-    acc[1] = fmaf(mr3, dy, acc[1]);
-    acc[2] = fmaf(mr3, dz, acc[2]);
-    acc[3] = fmaf(mr3, fmaf(al, dx, ux), acc[3]);
-    acc[4] = fmaf(mr3, fmaf(al, dy, uy), acc[4]);
-    acc[5] = fmaf(mr3, fmaf(al, dz, uz), acc[5]);
 }
 
 template <int BLOCK, bool MASK>

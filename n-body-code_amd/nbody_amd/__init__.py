@@ -75,6 +75,17 @@ def lib():
     L.nbh_launch_shape.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.nbh_device_info.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
     L.nbh_plummer.argtypes = [C.c_int, C.c_uint64] + [_D] * 7
+    _I = C.POINTER(C.c_int)
+    L.nbh_ens_create.argtypes = [vp, C.c_int, _I, _D, C.POINTER(vp)]
+    L.nbh_ens_destroy.argtypes = [vp]
+    L.nbh_ens_upload.argtypes = [vp] + [_D] * 7
+    L.nbh_ens_set_dt.argtypes = [vp, C.c_double]
+    L.nbh_ens_step.argtypes = [vp, C.c_int, C.c_int]
+    L.nbh_ens_download.argtypes = [vp] + [_D] * 13
+    L.nbh_ens_energy.argtypes = [vp, _D, _D]
+    L.nbh_ens_autotune.argtypes = [vp, _I, _I, _D]
+    L.nbh_ens_set_tuning.argtypes = [vp, _I, _I]
+    L.nbh_ens_info.argtypes = [vp, _I, _I, _I]
     L.nbh_reference_ic.argtypes = [C.c_int] + [C.c_double] * 4 + [_D] * 7
     _lib = L
     return L
@@ -228,3 +239,115 @@ class Backend:
 
     def reset_timing(self):
         _check(lib().nbh_reset_timing(self._h))
+
+
+# ---------------------------------------------------------------- ensemble --
+ENS_BUCKETS = 4
+ENS_BUCKET_EDGES = (64, 256, 1024, None)
+
+
+def ensemble_sizes(nsys, nmin, nmax, seed):
+    """Log-uniform integer system sizes in [nmin, nmax]:
+    n_k = floor(nmin * ((nmax + 1) / nmin) ** u_k), u_k ~ U[0, 1) from
+    numpy's PCG64 seeded with `seed`."""
+    u = np.random.Generator(np.random.PCG64(seed)).random(nsys)
+    n = np.floor(nmin * ((nmax + 1) / nmin) ** u).astype(np.int64)
+    return np.clip(n, nmin, nmax).astype(np.int32)
+
+
+def ensemble_state(counts, seed):
+    """Concatenated Plummer spheres, system k from nbh_plummer(n_k, seed*1000003 + k),
+    each in its own Henon units (total mass 1).  Returns (7, sum(counts))."""
+    counts = np.asarray(counts, dtype=np.int64)
+    out = np.empty((7, int(counts.sum())))
+    o = 0
+    L = lib()
+    for k, n in enumerate(counts):
+        blk = np.empty((7, int(n)))
+        rc = L.nbh_plummer(int(n), int(seed) * 1000003 + k, *[_p(blk[i]) for i in range(7)])
+        if rc != OK:
+            raise NBodyError(rc, f"nbh_plummer(n={n}) failed")
+        out[:, o:o + n] = blk
+        o += n
+    return out
+
+
+class Ensemble:
+    """nbh_ensemble: independent systems advanced by one persistent launch."""
+
+    def __init__(self, backend, counts, eps2):
+        self.counts = np.ascontiguousarray(counts, dtype=np.int32)
+        self.eps2 = np.ascontiguousarray(np.broadcast_to(eps2, self.counts.shape), dtype=np.float64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(np.int64)
+        self.total = int(self.counts.sum())
+        self.backend = backend
+        h = C.c_void_p()
+        _check(lib().nbh_ens_create(backend._h, len(self.counts),
+                                    self.counts.ctypes.data_as(C.POINTER(C.c_int)),
+                                    _p(self.eps2), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nbh_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, state):
+        s = _state(state)
+        if s.shape[1] != self.total:
+            raise ValueError(f"state has {s.shape[1]} bodies, ensemble {self.total}")
+        _check(lib().nbh_ens_upload(self._h, *[_p(s[k]) for k in range(7)]))
+
+    def set_dt(self, dt):
+        _check(lib().nbh_ens_set_dt(self._h, float(dt)))
+
+    def step(self, nsteps=1, graph=False):
+        _check(lib().nbh_ens_step(self._h, int(nsteps), int(bool(graph))))
+
+    def download(self):
+        s = np.empty((7, self.total))
+        f = np.empty((6, self.total))
+        _check(lib().nbh_ens_download(self._h, *[_p(s[k]) for k in range(7)], *[_p(f[k]) for k in range(6)]))
+        return s, f
+
+    def energy(self):
+        k = np.empty(len(self.counts))
+        u = np.empty(len(self.counts))
+        _check(lib().nbh_ens_energy(self._h, _p(k), _p(u)))
+        return k, u
+
+    def autotune(self):
+        t = np.zeros(ENS_BUCKETS, dtype=np.int32)
+        sp = np.zeros(ENS_BUCKETS, dtype=np.int32)
+        ms = np.zeros(ENS_BUCKETS)
+        _check(lib().nbh_ens_autotune(self._h, t.ctypes.data_as(C.POINTER(C.c_int)),
+                                      sp.ctypes.data_as(C.POINTER(C.c_int)), _p(ms)))
+        return {"tile": t.tolist(), "jsplit": sp.tolist(), "bucket_ms": ms.tolist()}
+
+    def set_tuning(self, tiles, jsplits):
+        t = np.ascontiguousarray(tiles, dtype=np.int32)
+        sp = np.ascontiguousarray(jsplits, dtype=np.int32)
+        _check(lib().nbh_ens_set_tuning(self._h, t.ctypes.data_as(C.POINTER(C.c_int)),
+                                        sp.ctypes.data_as(C.POINTER(C.c_int))))
+
+    def info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().nbh_ens_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"items": a.value, "workgroups": b.value, "max_tile": c.value}
+
+    def system(self, arr, k):
+        """Slice of a concatenated (c, total) array belonging to system k."""
+        o = self.offsets[k]
+        return arr[:, o:o + self.counts[k]]

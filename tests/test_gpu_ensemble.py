@@ -1,0 +1,171 @@
+"""GPU parity of the batched ensemble (persistent work-queue force kernel,
+size-bucket tuning, shared-dt Hermite steps, hipGraph replay) against the
+oracle run system by system, through the C ABI."""
+import numpy as np
+import pytest
+
+from conftest import body_relerr
+
+pytestmark = pytest.mark.gpu
+
+EDGE_SIZES = [1, 2, 3, 63, 64, 65, 128, 255, 256, 257, 700, 1025]
+
+
+def _banner(msg):
+    print(f"[gpu-ensemble] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def small(nb):
+    counts = np.concatenate([EDGE_SIZES, nb.ensemble_sizes(28, 4, 400, 7)]).astype(np.int32)
+    eps2 = (1.0 / (4.0 * counts.astype(float) ** (1 / 3))) ** 2
+    state = nb.ensemble_state(counts, 11)
+    return counts, eps2, state
+
+
+def check_systems(ens, f, state, oracle, eps2, tol_max=1e-12, mixed=False):
+    worst = 0.0
+    meds = []
+    for k in range(len(ens.counts)):
+        s = ens.system(state, k)
+        ref = oracle.force(s, eps2[k])
+        fk = ens.system(f, k)
+        if s.shape[1] == 1:
+            assert np.all(fk == 0.0)
+            continue
+        ea = body_relerr(fk[:3], ref[:3])
+        if mixed:
+            meds.append(np.median(ea))
+        else:
+            ej = body_relerr(fk[3:], ref[3:])
+            worst = max(worst, ea.max(), ej.max())
+    if mixed:
+        assert np.median(meds) <= 1e-6 and max(meds) <= 1e-5, (np.median(meds), max(meds))
+    else:
+        assert worst <= tol_max, worst
+    return worst
+
+
+def test_initial_forces_fp64(nb, oracle, small):
+    counts, eps2, state = small
+    with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.upload(state)
+        _, f = e.download()
+        w = check_systems(e, f, state, oracle, eps2)
+        _banner(f"fp64 ensemble of {len(counts)} systems: worst per-body rel err {w:.2e}")
+
+
+def test_initial_forces_mixed(nb, oracle, small):
+    counts, eps2, state = small
+    with nb.Backend("mixed") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.upload(state)
+        _, f = e.download()
+        check_systems(e, f, state, oracle, eps2, mixed=True)
+
+
+@pytest.mark.parametrize("tile", [64, 128, 256])
+@pytest.mark.parametrize("split", [1, 2, 4, 8])
+def test_tunings_agree(nb, oracle, small, tile, split):
+    counts, eps2, state = small
+    with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.set_tuning([tile] * 4, [split] * 4)
+        e.upload(state)
+        _, f = e.download()
+        check_systems(e, f, state, oracle, eps2)
+
+
+def test_tile_length_does_not_change_sums(nb, small):
+    """With no j-split every target sums its sources in index order, so the
+    LDS tile length cannot change the FP64 or mixed result."""
+    counts, eps2, state = small
+    for prec in ("fp64", "mixed"):
+        outs = []
+        with nb.Backend(prec) as b, nb.Ensemble(b, counts, eps2) as e:
+            for tile in (64, 128, 256):
+                e.set_tuning([tile] * 4, [1] * 4)
+                e.upload(state)
+                outs.append(e.download()[1])
+        assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_steps_vs_oracle(nb, oracle, small):
+    counts, eps2, state = small
+    dt, nsteps = 1 / 256, 4
+    with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.set_tuning([64, 128, 128, 256], [1, 2, 2, 4])
+        e.upload(state)
+        e.set_dt(dt)
+        e.step(nsteps)
+        st, f = e.download()
+    worst = 0.0
+    for k in range(len(counts)):
+        o = oracle.run(e.system(state, k), eps2[k], dt, nsteps)
+        worst = max(worst, np.abs(e.system(st, k) - o["state"]).max())
+    _banner(f"4 ensemble steps: worst state difference {worst:.2e}")
+    assert worst <= 1e-12
+
+
+def test_graph_matches_eager(nb, small):
+    counts, eps2, state = small
+    res = []
+    for graph in (False, True):
+        with nb.Backend("mixed") as b, nb.Ensemble(b, counts, eps2) as e:
+            e.set_tuning([64, 128, 128, 256], [1, 2, 4, 8])
+            e.upload(state)
+            e.set_dt(1 / 256)
+            e.step(20, graph=graph)   # one 16-step graph + 4 eager steps
+            e.step(32, graph=graph)   # the graph replayed twice
+            res.append(e.download())
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
+def test_energy_per_system(nb, oracle, small):
+    counts, eps2, state = small
+    with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.upload(state)
+        kin, pot = e.energy()
+    for k in range(len(counts)):
+        s = e.system(state, k)
+        assert kin[k] == pytest.approx(oracle.kinetic(s), rel=1e-13)
+        if counts[k] > 1:
+            assert pot[k] == pytest.approx(oracle.potential(s, eps2[k]), rel=1e-12)
+        else:
+            assert pot[k] == 0.0
+
+
+def test_autotune_then_parity(nb, oracle, small):
+    counts, eps2, state = small
+    with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.upload(state)
+        t = e.autotune()
+        _banner(f"autotune: {t}")
+        assert all(x in (64, 128, 256) for x in t["tile"])
+        assert all(x in (1, 2, 4, 8) for x in t["jsplit"])
+        e.upload(state)
+        _, f = e.download()
+        check_systems(e, f, state, oracle, eps2)
+
+
+def test_zero_softening_system(nb, oracle):
+    counts = np.array([50, 130], dtype=np.int32)
+    eps2 = np.array([0.0, 1e-3])
+    state = nb.ensemble_state(counts, 5)
+    with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.upload(state)
+        _, f = e.download()
+        check_systems(e, f, state, oracle, eps2)
+
+
+def test_bad_arguments(nb):
+    with nb.Backend("fp64") as b:
+        with pytest.raises(nb.NBodyError) as ex:
+            nb.Ensemble(b, [3, 0, 4], 1e-3)
+        assert ex.value.code == nb.ERR_ARG
+        with nb.Ensemble(b, [3, 4], 1e-3) as e:
+            with pytest.raises(nb.NBodyError) as ex:
+                e.step(1)
+            assert ex.value.code == nb.ERR_STATE
+            with pytest.raises(nb.NBodyError) as ex:
+                e.set_tuning([96, 64, 64, 64], [1, 1, 1, 1])
+            assert ex.value.code == nb.ERR_ARG
