@@ -113,6 +113,38 @@ def run_ensemble(nb, log, nsteps=64):
     return out
 
 
+def run_adjoint(nb, log):
+    """Differentiation workload: N=4096 Plummer (seed 5), masses x
+    lognormal(0.1) renormalised to 1, eps=1/64, dt=1/512, 128 FP64 steps,
+    checkpoints every 16; loss = sum |x(T) - x_target|^2 with targets from a
+    run with re-drawn masses (seed 6).  Times the forward run and the
+    checkpointed backward sweep (which recomputes each segment)."""
+    n, dt, nsteps, every = 4096, 1.0 / 512, 128, 16
+    s = nb.plummer(n, 5)
+    m = s[3] * np.random.Generator(np.random.PCG64(5)).lognormal(0.0, 0.1, n)
+    s[3] = m / m.sum()
+    st = s.copy()
+    mt = np.random.Generator(np.random.PCG64(6)).lognormal(0.0, 0.1, n)
+    st[3] = mt / mt.sum()
+    with nb.Backend("fp64", (1.0 / 64) ** 2) as b, nb.Adjoint(b, n, every) as adj:
+        target = adj.forward(st, dt, nsteps)[0:3]
+        t0 = time.perf_counter()
+        fin = adj.forward(s, dt, nsteps)
+        t1 = time.perf_counter()
+        cot = np.zeros((7, n))
+        cot[0:3] = 2.0 * (fin[0:3] - target)
+        g = adj.backward(cot)
+        t2 = time.perf_counter()
+    out = {"workload": "N=4096 Plummer seed 5, lognormal(0.1) masses, eps=1/64, dt=1/512, 128 FP64 steps, "
+                       "checkpoints every 16",
+           "forward_s": t1 - t0, "backward_s": t2 - t1,
+           "loss": float(((fin[0:3] - target) ** 2).sum()),
+           "grad_norm": {"x": float(np.linalg.norm(g[0:3])), "v": float(np.linalg.norm(g[4:7])),
+                         "m": float(np.linalg.norm(g[3]))}}
+    log(f"adjoint: forward {out['forward_s'] * 1e3:.1f} ms, backward {out['backward_s'] * 1e3:.1f} ms")
+    return out
+
+
 def cpu_baseline(state, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
@@ -162,6 +194,7 @@ def main():
     log(f"mixed vs fp64 acc rel err: median {med:.2e}, max {mx:.2e}")
 
     ens = None if args.no_ensemble else run_ensemble(nb, log)
+    adj = None if args.no_ensemble else run_adjoint(nb, log)
 
     cpu = None
     if not args.no_cpu_baseline and rank == 0:
@@ -201,6 +234,7 @@ def main():
                           "flop_per_launch": FLOP_PER_PAIR * N * N},
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "adjoint": adj,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -204,6 +204,33 @@ int nbh_ens_set_tuning(nbh_ensemble *ens, const int *tiles, const int *jsplits);
 /* Work items per force evaluation, persistent workgroups, largest tile. */
 int nbh_ens_info(nbh_ensemble *ens, int *items, int *workgroups, int *max_tile);
 
+/* ---- reverse-mode derivatives (FP64) ---------------------------------- */
+/* No counterpart in the reference.  Arrays are blocked: a state is [7][n]
+ * (x, y, z, m, vx, vy, vz blocks, the order of force_calculation's
+ * arguments), forces and their cotangents are [6][n] (ax, ay, az, jx, jy,
+ * jz).  The context must be FP64; its eps2 is the softening. */
+
+/* Vector-Jacobian product of the force evaluation (the pair sum of
+ * src/kernel.cpp:1164-1224): grad = d/d(state) of
+ * sum_i cot_a,i . a_i + cot_j,i . jerk_i, as a [7][n] array. */
+int nbh_force_vjp(nbh_ctx *ctx, int n, const double *state, const double *cot,
+                  double *grad);
+
+/* Adjoint of a fixed-dt Hermite trajectory (NBody::timestepping,
+ * src/nbody.cpp:224-289, repeated).  The forward run keeps the carried
+ * state (positions, velocities, forces) every `every` steps; the backward
+ * sweep recomputes each segment from its checkpoint and runs the step
+ * adjoints in reverse, then the adjoint of the initial force evaluation. */
+typedef struct nbh_adjoint nbh_adjoint;
+int nbh_adj_create(nbh_ctx *ctx, int n, int every, nbh_adjoint **out);
+int nbh_adj_destroy(nbh_adjoint *adj);
+/* state0 [7][n] -> state_final [7][n] (may be NULL) after nsteps steps. */
+int nbh_adj_forward(nbh_adjoint *adj, const double *state0, double dt,
+                    int nsteps, double *state_final);
+/* cot_final [7][n]: cotangent of the final state (the m block adds to the
+ * mass gradient directly) -> grad0 [7][n]: gradient w.r.t. state0. */
+int nbh_adj_backward(nbh_adjoint *adj, const double *cot_final, double *grad0);
+
 /* ---- initial conditions (host code, no device needed) --------------- */
 
 /* Equal-mass Plummer sphere in Henon units (G = M = 1, E = -1/4):

@@ -1,0 +1,482 @@
+// nbh_adjoint.hip -- reverse-mode derivatives of the force evaluation and of
+// the Hermite trajectory (FP64).
+//
+// Force VJP.  The forward map is, per directed pair (target i, source j),
+//   a_i    += m_j q d,                      q = s^-3/2, s = |d|^2 + eps2
+//   jerk_i += m_j (q u - 3 (d.u) p d),      p = s^-5/2, d = x_j - x_i, u = v_j - v_i
+// (src/kernel.cpp:1188-1205).  Given cotangents (alpha_i, beta_i) of (a_i,
+// jerk_i), a pair contributes to body k both as a target and as a source.
+// With G(A, B; d, u) = q A - 3p (A.d + B.u) d - 3p ((B.d) u + (d.u) B)
+//                      + 15 (d.u)(B.d) s^-7/2 d
+// and H(B; d) = q B - 3p (B.d) d, both even in (d, u) and linear in (A, B),
+// the two roles combine per pair (k, o) into
+//   xbar_k += G(m_k alpha_o - m_o alpha_k, m_k beta_o - m_o beta_k)
+//   vbar_k += H(m_k beta_o - m_o beta_k)
+//   mbar_k += -q (alpha_o.d + beta_o.u) + 3 (d.u) p (beta_o.d)
+// with d = x_o - x_k, u = v_o - v_k: a gather over o, the same streaming
+// pattern as the force (one target per lane, sources through LDS, j-split
+// with an ordered second pass), no atomics.
+//
+// Step VJP.  One step (src/nbody.cpp:152-196, fixed dt) is, with
+// (a1, j1) = F(x_p, v_p, m),
+//   x_p = x + v dt + a dt^2/2 + j dt^3/6,   v_p = v + a dt + j dt^2/2
+//   x'  = x_p - 0.15 dt^2 (a - a1) - 7/60 dt^3 j - 1/30 dt^3 j1
+//   v'  = v_p - 0.5 dt (a - a1) - 5/12 dt^2 j - 1/12 dt^2 j1
+// and carries (x', v', a1, j1); the corrector's snap/crackle algebra
+// reduces to those coefficients.  Its adjoint is the O(n) kernels k_adj_post
+// and k_adj_pre around one force VJP.  Trajectories keep checkpoints every
+// `every` steps and recompute each segment forward during the backward sweep.
+#include "nbh_ctx.h"
+#include "nbh_pair.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace nbh {
+namespace {
+
+constexpr int kVB = 64;       // targets per workgroup of the VJP kernel
+constexpr int kOB = 256;      // threads per workgroup of the O(n) kernels
+
+// Source record in LDS: position, mass, velocity, and the cotangents of the
+// source's acceleration and jerk.
+struct __align__(16) VJB {
+    double x, y, z, m;
+    double vx, vy, vz, p0;
+    double ax, ay, az, p1;
+    double jx, jy, jz, p2;
+};
+
+struct VjpArgs {
+    State s;
+    const double* cot;  // [6][n]: alpha (3 blocks), beta (3 blocks)
+    double* out;        // [split][7][n]: xbar(3), mbar, vbar(3)
+    int n;
+    int jchunk;
+    double eps2;
+};
+
+template <bool MASK>
+__global__ __launch_bounds__(kVB) void k_force_vjp(VjpArgs a)
+{
+    __shared__ VJB tile[kVB];
+    const int t = threadIdx.x;
+    const int k = blockIdx.x * kVB + t;
+    const int n = a.n;
+    const int kc = k < n ? k : n - 1;
+    const double xk = a.s.x[kc], yk = a.s.y[kc], zk = a.s.z[kc], mk = a.s.m[kc];
+    const double vxk = a.s.vx[kc], vyk = a.s.vy[kc], vzk = a.s.vz[kc];
+    const double axk = a.cot[kc], ayk = a.cot[n + kc], azk = a.cot[2 * (size_t)n + kc];
+    const double bxk = a.cot[3 * (size_t)n + kc], byk = a.cot[4 * (size_t)n + kc], bzk = a.cot[5 * (size_t)n + kc];
+    double gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0, gm = 0;
+    const int jbeg = blockIdx.y * a.jchunk;
+    const int jend = min(n, jbeg + a.jchunk);
+    for (int jt = jbeg; jt < jend; jt += kVB) {
+        const int cnt = min(kVB, jend - jt);
+        __syncthreads();
+        if (t < cnt) {
+            const int o = jt + t;
+            VJB b;
+            b.x = a.s.x[o]; b.y = a.s.y[o]; b.z = a.s.z[o]; b.m = a.s.m[o];
+            b.vx = a.s.vx[o]; b.vy = a.s.vy[o]; b.vz = a.s.vz[o];
+            b.ax = a.cot[o]; b.ay = a.cot[n + o]; b.az = a.cot[2 * (size_t)n + o];
+            b.jx = a.cot[3 * (size_t)n + o]; b.jy = a.cot[4 * (size_t)n + o]; b.jz = a.cot[5 * (size_t)n + o];
+            b.p0 = b.p1 = b.p2 = 0.0;
+            tile[t] = b;
+        }
+        __syncthreads();
+        for (int q0 = 0; q0 < cnt; ++q0) {
+            const VJB& b = tile[q0];
+            const double dx = b.x - xk, dy = b.y - yk, dz = b.z - zk;
+            const double ux = b.vx - vxk, uy = b.vy - vyk, uz = b.vz - vzk;
+            double s = fma(dx, dx, fma(dy, dy, fma(dz, dz, a.eps2)));
+            double mo = b.m, mkk = mk;
+            if (MASK) {
+                const bool self = (jt + q0) == k;
+                s = self ? 1.0 : s;
+                mo = self ? 0.0 : mo;
+                mkk = self ? 0.0 : mkk;
+            }
+            const double r = rsqrt_f64(s);
+            const double r2 = r * r;
+            const double q = r * r2;
+            const double p = q * r2;
+            const double ps = p * r2;
+            const double du = fma(dx, ux, fma(dy, uy, dz * uz));
+            // combined cotangents of the pair
+            const double Ax = fma(mkk, b.ax, -mo * axk), Ay = fma(mkk, b.ay, -mo * ayk), Az = fma(mkk, b.az, -mo * azk);
+            const double Bx = fma(mkk, b.jx, -mo * bxk), By = fma(mkk, b.jy, -mo * byk), Bz = fma(mkk, b.jz, -mo * bzk);
+            const double Ad = fma(Ax, dx, fma(Ay, dy, Az * dz));
+            const double Bu = fma(Bx, ux, fma(By, uy, Bz * uz));
+            const double Bd = fma(Bx, dx, fma(By, dy, Bz * dz));
+            const double p3 = -3.0 * p;
+            const double c1 = fma(15.0 * du * Bd, ps, p3 * (Ad + Bu));
+            gx = fma(q, Ax, fma(c1, dx, fma(p3, fma(Bd, ux, du * Bx), gx)));
+            gy = fma(q, Ay, fma(c1, dy, fma(p3, fma(Bd, uy, du * By), gy)));
+            gz = fma(q, Az, fma(c1, dz, fma(p3, fma(Bd, uz, du * Bz), gz)));
+            const double e = p3 * Bd;
+            hx = fma(q, Bx, fma(e, dx, hx));
+            hy = fma(q, By, fma(e, dy, hy));
+            hz = fma(q, Bz, fma(e, dz, hz));
+            // mass: the source role only (force is linear in the source mass)
+            const double ad = fma(b.ax, dx, fma(b.ay, dy, b.az * dz));
+            const double bu = fma(b.jx, ux, fma(b.jy, uy, b.jz * uz));
+            const double bd = fma(b.jx, dx, fma(b.jy, dy, b.jz * dz));
+            const double gmo = fma(3.0 * du * p, bd, -q * (ad + bu));
+            gm += MASK ? ((jt + q0) == k ? 0.0 : gmo) : gmo;
+        }
+    }
+    if (k < n) {
+        double* o = a.out + (size_t)blockIdx.y * 7 * n;
+        o[k] = gx;
+        o[n + k] = gy;
+        o[2 * (size_t)n + k] = gz;
+        o[3 * (size_t)n + k] = gm;
+        o[4 * (size_t)n + k] = hx;
+        o[5 * (size_t)n + k] = hy;
+        o[6 * (size_t)n + k] = hz;
+    }
+}
+
+// grad[c][k] (+)= sum over splits, in split order.
+__global__ __launch_bounds__(kOB) void k_vjp_reduce(const double* __restrict__ part, int n, int splits,
+                                                    double* grad, int accumulate)
+{
+    const int k = blockIdx.x * kOB + threadIdx.x;
+    const int c = blockIdx.y;
+    if (k >= n) return;
+    double v = 0.0;
+    for (int s = 0; s < splits; ++s) v += part[((size_t)s * 7 + c) * n + k];
+    const size_t o = (size_t)c * n + k;
+    grad[o] = accumulate ? grad[o] + v : v;
+}
+
+// Step adjoint, first half: from the cotangents of the step's outputs
+// (xbar', vbar', abar', jbar' in `cot` = [12][n]: x(3) v(3) a(3) j(3)),
+// writes the cotangents of (a1, j1) into cf = [6][n], keeps xbar_p = xbar',
+// vbar_p = vbar' in cot[0..5], and puts the direct part of (abar, jbar)
+// into cot[6..11].
+__global__ __launch_bounds__(kOB) void k_adj_post(int n, double dt, double* cot, double* cf)
+{
+    const int i = blockIdx.x * kOB + threadIdx.x;
+    if (i >= n) return;
+    const double ca = -0.15 * dt * dt, cj = -7.0 / 60.0 * dt * dt * dt, cj1 = -1.0 / 30.0 * dt * dt * dt;
+    const double da = -0.5 * dt, dj = -5.0 / 12.0 * dt * dt, dj1 = -1.0 / 12.0 * dt * dt;
+    for (int c = 0; c < 3; ++c) {
+        const size_t X = (size_t)c * n + i, V = (size_t)(3 + c) * n + i;
+        const size_t A = (size_t)(6 + c) * n + i, J = (size_t)(9 + c) * n + i;
+        const double xb = cot[X], vb = cot[V];
+        cf[(size_t)c * n + i] = cot[A] - ca * xb - da * vb;        // abar of a1
+        cf[(size_t)(3 + c) * n + i] = cot[J] + cj1 * xb + dj1 * vb; // jbar of j1
+        cot[A] = ca * xb + da * vb;                                 // abar of a (direct)
+        cot[J] = cj * xb + dj * vb;                                 // jbar of j (direct)
+    }
+}
+
+// Step adjoint, second half: adds the force VJP (g = [7][n]: x(3) m v(3)) to
+// (xbar_p, vbar_p), accumulates mbar, and pulls the cotangents back through
+// the predictor onto the step's inputs (x, v, a, j).
+__global__ __launch_bounds__(kOB) void k_adj_pre(int n, double dt, double* cot, const double* g, double* mbar)
+{
+    const int i = blockIdx.x * kOB + threadIdx.x;
+    if (i >= n) return;
+    mbar[i] += g[3 * (size_t)n + i];
+    for (int c = 0; c < 3; ++c) {
+        const size_t X = (size_t)c * n + i, V = (size_t)(3 + c) * n + i;
+        const size_t A = (size_t)(6 + c) * n + i, J = (size_t)(9 + c) * n + i;
+        const double xp = cot[X] + g[(size_t)c * n + i];
+        const double vp = cot[V] + g[(size_t)(4 + c) * n + i];
+        cot[X] = xp;
+        cot[V] = fma(xp, dt, vp);
+        cot[A] = cot[A] + xp * (0.5 * dt * dt) + vp * dt;
+        cot[J] = cot[J] + xp * (dt * dt * dt / 6.0) + vp * (0.5 * dt * dt);
+    }
+}
+
+inline int oblk(int n) { return (n + kOB - 1) / kOB; }
+
+}  // namespace
+
+// Force VJP on device arrays: grad = [7][n] (x(3), m, v(3)) of
+// sum_i alpha_i.a_i + beta_i.jerk_i, cot = [6][n].  accumulate adds to grad.
+hipError_t launch_force_vjp(const State& s, int n, double eps2, const double* cot, double* grad,
+                            int accumulate, double* partial, int splits, hipStream_t st)
+{
+    VjpArgs a;
+    a.s = s;
+    a.cot = cot;
+    a.out = partial;
+    a.n = n;
+    const int tiles = (n + kVB - 1) / kVB;
+    const int per = (tiles + splits - 1) / splits;
+    a.jchunk = per * kVB;
+    const int S = (n + a.jchunk - 1) / a.jchunk;
+    a.eps2 = eps2;
+    const dim3 grid(tiles, S);
+    if (eps2 == 0.0)
+        k_force_vjp<true><<<grid, kVB, 0, st>>>(a);
+    else
+        k_force_vjp<false><<<grid, kVB, 0, st>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_vjp_reduce<<<dim3(oblk(n), 7), kOB, 0, st>>>(partial, n, S, grad, accumulate);
+    return hipGetLastError();
+}
+
+// j-split of the FP64 force evaluations inside the trajectory (64-lane
+// workgroups, about 8 per CU).
+int adj_force_splits(int n, int cus)
+{
+    const int tiles = (n + 63) / 64;
+    return std::max(1, std::min(tiles, (8 * (cus > 0 ? cus : 256) + tiles - 1) / tiles));
+}
+
+int vjp_splits(int n, int cus)
+{
+    const int tiles = (n + kVB - 1) / kVB;
+    const int want = 4 * (cus > 0 ? cus : 256);
+    int s = (want + tiles - 1) / tiles;
+    return std::max(1, std::min(s, tiles));
+}
+
+}  // namespace nbh
+
+using nbh::DevBuf;
+using nbh::fail;
+
+struct nbh_adjoint {
+    nbh_ctx* ctx = nullptr;
+    int n = 0;
+    int every = 1;
+    int nsteps = 0;
+    double dt = 0.0;
+    bool have_forward = false;
+    DevBuf ckpt;     // [ncp][13][n]: state(7) forces(6)
+    DevBuf seg;      // [every][13][n]
+    DevBuf cur;      // [13][n]
+    DevBuf f1;       // [6][n]
+    DevBuf pred;     // [7][n]
+    DevBuf cot;      // [12][n]
+    DevBuf cf;       // [6][n]
+    DevBuf g;        // [7][n]
+    DevBuf mbar;     // [n]
+    DevBuf partial;  // [splits][7][n] (also the force split partials)
+    DevBuf dtbuf;    // [1]
+};
+
+namespace {
+
+int splits_of(const nbh_adjoint* a) { return nbh::vjp_splits(a->n, a->ctx->cus); }
+
+// One FP64 force evaluation of the 7n state at `st` into `f` (6n).
+int adj_force(nbh_adjoint* a, double* st, double* f)
+{
+    nbh_ctx* c = a->ctx;
+    const int n = a->n;
+    nbh::Shape sh{};
+    sh.block = 64;
+    const int tiles = (n + 63) / 64;
+    const int split = nbh::adj_force_splits(n, c->cus);
+    const int per = (tiles + split - 1) / split;
+    sh.jchunk = per * 64;
+    sh.jsplit = (n + sh.jchunk - 1) / sh.jchunk;
+    hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(st, n), n, c->cfg.eps2, sh, nbh::forces_of(f, n),
+                                     a->partial.d(), c->stream, nullptr, nullptr);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "adjoint force launch failed: %s", hipGetErrorString(e));
+    return NBH_OK;
+}
+
+// One forward Hermite step of cur (state + forces) in place.
+int adj_step(nbh_adjoint* a)
+{
+    nbh_ctx* c = a->ctx;
+    const int n = a->n;
+    double* b = a->cur.d();
+    const nbh::Forces f0 = nbh::forces_of(b + 7 * (size_t)n, n);
+    const nbh::Forces f1 = nbh::forces_of(a->f1.d(), n);
+    NBH_HIP(nbh::launch_predict(n, a->dtbuf.d(), b, b + n, b + 2 * (size_t)n, b + 4 * (size_t)n, b + 5 * (size_t)n,
+                                b + 6 * (size_t)n, f0, c->stream));
+    int rc = adj_force(a, b, a->f1.d());
+    if (rc) return rc;
+    NBH_HIP(nbh::launch_correct(n, a->dtbuf.d(), b, b + n, b + 2 * (size_t)n, b + 4 * (size_t)n, b + 5 * (size_t)n,
+                                b + 6 * (size_t)n, f0, f1, nullptr, 0.0, c->stream));
+    return NBH_OK;
+}
+
+size_t sz(int n, int k) { return sizeof(double) * (size_t)k * (size_t)n; }
+
+}  // namespace
+
+extern "C" {
+
+int nbh_force_vjp(nbh_ctx* c, int n, const double* state, const double* cot, double* grad)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (n < 1) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    if (!state || !cot || !grad) return fail(NBH_ERR_ARG, "state/cot/grad array is NULL");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const int splits = nbh::vjp_splits(n, c->cus);
+    DevBuf ds, dc, dg, dp;
+    if (ds.reserve(sz(n, 7)) != hipSuccess || dc.reserve(sz(n, 6)) != hipSuccess ||
+        dg.reserve(sz(n, 7)) != hipSuccess || dp.reserve(sz(n, 7 * splits)) != hipSuccess) {
+        ds.release(); dc.release(); dg.release(); dp.release();
+        return fail(NBH_ERR_ALLOC, "device allocation for the force VJP failed");
+    }
+    int rc = NBH_OK;
+    hipError_t e = hipMemcpyAsync(ds.p, state, sz(n, 7), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dc.p, cot, sz(n, 6), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = nbh::launch_force_vjp(nbh::state_of(ds.d(), n), n, c->cfg.eps2, dc.d(), dg.d(), 0, dp.d(), splits,
+                                  c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(grad, dg.p, sz(n, 7), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(NBH_ERR_HIP, "force VJP failed: %s", hipGetErrorString(e));
+    ds.release(); dc.release(); dg.release(); dp.release();
+    return rc;
+}
+
+int nbh_adj_create(nbh_ctx* c, int n, int every, nbh_adjoint** out)
+{
+    if (!out) return fail(NBH_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (n < 1) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    if (every < 1) return fail(NBH_ERR_ARG, "checkpoint interval must be >= 1, got %d", every);
+    if (c->cfg.precision != NBH_FP64) return fail(NBH_ERR_ARG, "the adjoint needs an FP64 context");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    auto* a = new (std::nothrow) nbh_adjoint();
+    if (!a) return fail(NBH_ERR_ALLOC, "adjoint allocation failed");
+    a->ctx = c;
+    a->n = n;
+    a->every = every;
+    // partial sums of both the force j-split (6 planes per split) and the VJP
+    // j-split (7 planes per split)
+    const int planes = std::max(6 * nbh::adj_force_splits(n, c->cus), 7 * splits_of(a));
+    if (a->seg.reserve(sz(n, 13 * every)) != hipSuccess || a->cur.reserve(sz(n, 13)) != hipSuccess ||
+        a->f1.reserve(sz(n, 6)) != hipSuccess || a->pred.reserve(sz(n, 7)) != hipSuccess ||
+        a->cot.reserve(sz(n, 12)) != hipSuccess || a->cf.reserve(sz(n, 6)) != hipSuccess ||
+        a->g.reserve(sz(n, 7)) != hipSuccess || a->mbar.reserve(sz(n, 1)) != hipSuccess ||
+        a->partial.reserve(sz(n, planes)) != hipSuccess || a->dtbuf.reserve(sizeof(double)) != hipSuccess) {
+        nbh_adj_destroy(a);
+        return fail(NBH_ERR_ALLOC, "adjoint device allocation failed (n=%d, every=%d)", n, every);
+    }
+    *out = a;
+    return NBH_OK;
+}
+
+int nbh_adj_destroy(nbh_adjoint* a)
+{
+    if (!a) return NBH_OK;
+    (void)hipSetDevice(a->ctx->cfg.device);
+    (void)hipStreamSynchronize(a->ctx->stream);
+    for (DevBuf* b : {&a->ckpt, &a->seg, &a->cur, &a->f1, &a->pred, &a->cot, &a->cf, &a->g, &a->mbar, &a->partial,
+                      &a->dtbuf})
+        b->release();
+    delete a;
+    return NBH_OK;
+}
+
+int nbh_adj_forward(nbh_adjoint* a, const double* state0, double dt, int nsteps, double* state_final)
+{
+    if (!a) return fail(NBH_ERR_ARG, "adjoint is NULL");
+    if (!state0) return fail(NBH_ERR_ARG, "state0 is NULL");
+    if (nsteps < 0) return fail(NBH_ERR_ARG, "nsteps must be >= 0");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(NBH_ERR_ARG, "dt must be finite and > 0");
+    nbh_ctx* c = a->ctx;
+    if (c->cfg.precision != NBH_FP64) return fail(NBH_ERR_ARG, "the adjoint needs an FP64 context");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const int n = a->n;
+    const int ncp = nsteps / a->every + 1;
+    if (a->ckpt.reserve(sz(n, 13 * ncp)) != hipSuccess) return fail(NBH_ERR_ALLOC, "checkpoint allocation failed");
+    a->have_forward = false;
+    a->nsteps = nsteps;
+    a->dt = dt;
+    NBH_HIP(hipMemcpyAsync(a->dtbuf.p, &a->dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    double* cur = a->cur.d();
+    NBH_HIP(hipMemcpyAsync(cur, state0, sz(n, 7), hipMemcpyHostToDevice, c->stream));
+    int rc = adj_force(a, cur, cur + 7 * (size_t)n);
+    if (rc) return rc;
+    for (int t = 0; t <= nsteps; ++t) {
+        if (t % a->every == 0)
+            NBH_HIP(hipMemcpyAsync(a->ckpt.d() + (size_t)(t / a->every) * 13 * n, cur, sz(n, 13),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        if (t == nsteps) break;
+        if ((rc = adj_step(a))) return rc;
+    }
+    if (state_final) NBH_HIP(hipMemcpyAsync(state_final, cur, sz(n, 7), hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    a->have_forward = true;
+    return NBH_OK;
+}
+
+int nbh_adj_backward(nbh_adjoint* a, const double* cot_final, double* grad0)
+{
+    if (!a) return fail(NBH_ERR_ARG, "adjoint is NULL");
+    if (!cot_final || !grad0) return fail(NBH_ERR_ARG, "cot_final/grad0 is NULL");
+    if (!a->have_forward) return fail(NBH_ERR_STATE, "no forward run: call nbh_adj_forward first");
+    nbh_ctx* c = a->ctx;
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const int n = a->n;
+    const int splits = splits_of(a);
+    const double dt = a->dt;
+    double* cot = a->cot.d();
+    // cotangent of the final carried state: x, v from the caller, a = j = 0
+    NBH_HIP(hipMemsetAsync(cot, 0, sz(n, 12), c->stream));
+    NBH_HIP(hipMemcpyAsync(cot, cot_final, sz(n, 3), hipMemcpyHostToDevice, c->stream));
+    NBH_HIP(hipMemcpyAsync(cot + 3 * (size_t)n, cot_final + 4 * (size_t)n, sz(n, 3), hipMemcpyHostToDevice,
+                           c->stream));
+    NBH_HIP(hipMemcpyAsync(a->mbar.p, cot_final + 3 * (size_t)n, sz(n, 1), hipMemcpyHostToDevice, c->stream));
+    const int nseg = (a->nsteps + a->every - 1) / a->every;
+    const int ob = nbh::oblk(n);
+    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+        const int s0 = sgi * a->every;
+        const int s1 = std::min(a->nsteps, s0 + a->every);
+        // recompute the segment from its checkpoint, keeping every step's input
+        NBH_HIP(hipMemcpyAsync(a->cur.p, a->ckpt.d() + (size_t)sgi * 13 * n, sz(n, 13), hipMemcpyDeviceToDevice,
+                               c->stream));
+        for (int t = s0; t < s1; ++t) {
+            NBH_HIP(hipMemcpyAsync(a->seg.d() + (size_t)(t - s0) * 13 * n, a->cur.p, sz(n, 13),
+                                   hipMemcpyDeviceToDevice, c->stream));
+            if (t + 1 < s1) {
+                int rc = adj_step(a);
+                if (rc) return rc;
+            }
+        }
+        for (int t = s1 - 1; t >= s0; --t) {
+            const double* in = a->seg.d() + (size_t)(t - s0) * 13 * n;
+            nbh::k_adj_post<<<ob, nbh::kOB, 0, c->stream>>>(n, dt, cot, a->cf.d());
+            NBH_HIP(hipGetLastError());
+            // predicted state of step t
+            double* p = a->pred.d();
+            NBH_HIP(hipMemcpyAsync(p, in, sz(n, 7), hipMemcpyDeviceToDevice, c->stream));
+            const nbh::Forces f0 = nbh::forces_of(const_cast<double*>(in) + 7 * (size_t)n, n);
+            NBH_HIP(nbh::launch_predict(n, a->dtbuf.d(), p, p + n, p + 2 * (size_t)n, p + 4 * (size_t)n,
+                                        p + 5 * (size_t)n, p + 6 * (size_t)n, f0, c->stream));
+            NBH_HIP(nbh::launch_force_vjp(nbh::state_of(p, n), n, c->cfg.eps2, a->cf.d(), a->g.d(), 0,
+                                          a->partial.d(), splits, c->stream));
+            nbh::k_adj_pre<<<ob, nbh::kOB, 0, c->stream>>>(n, dt, cot, a->g.d(), a->mbar.d());
+            NBH_HIP(hipGetLastError());
+        }
+    }
+    // the initial forces (a0, j0) = F(x0, v0, m)
+    double* x0 = a->ckpt.d();
+    NBH_HIP(nbh::launch_force_vjp(nbh::state_of(x0, n), n, c->cfg.eps2, cot + 6 * (size_t)n, a->g.d(), 0,
+                                  a->partial.d(), splits, c->stream));
+    // grad0 = (xbar + g_x, mbar + g_m, vbar + g_v)
+    std::vector<double> h((size_t)7 * n), hc((size_t)6 * n), hm(n);
+    NBH_HIP(hipMemcpyAsync(h.data(), a->g.p, sz(n, 7), hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipMemcpyAsync(hc.data(), cot, sz(n, 6), hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipMemcpyAsync(hm.data(), a->mbar.p, sz(n, 1), hipMemcpyDeviceToHost, c->stream));
+    NBH_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            grad0[(size_t)k * n + i] = hc[(size_t)k * n + i] + h[(size_t)k * n + i];
+            grad0[(size_t)(4 + k) * n + i] = hc[(size_t)(3 + k) * n + i] + h[(size_t)(4 + k) * n + i];
+        }
+        grad0[3 * (size_t)n + i] = hm[i] + h[3 * (size_t)n + i];
+    }
+    return NBH_OK;
+}
+
+}  // extern "C"

@@ -87,6 +87,11 @@ def lib():
     L.nbh_ens_set_tuning.argtypes = [vp, _I, _I]
     L.nbh_ens_info.argtypes = [vp, _I, _I, _I]
     L.nbh_reference_ic.argtypes = [C.c_int] + [C.c_double] * 4 + [_D] * 7
+    L.nbh_force_vjp.argtypes = [vp, C.c_int, _D, _D, _D]
+    L.nbh_adj_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+    L.nbh_adj_destroy.argtypes = [vp]
+    L.nbh_adj_forward.argtypes = [vp, _D, C.c_double, C.c_int, _D]
+    L.nbh_adj_backward.argtypes = [vp, _D, _D]
     _lib = L
     return L
 
@@ -228,6 +233,17 @@ class Backend:
                                      int(bool(compat)), C.byref(k), C.byref(u)))
         return k.value, u.value
 
+    # ---- derivatives ----
+    def force_vjp(self, state, cot):
+        """d/d(state) of sum(cot * forces): state (7, n), cot (6, n) -> (7, n)."""
+        s = _state(state)
+        c = np.ascontiguousarray(cot, dtype=np.float64)
+        if c.shape != (6, s.shape[1]):
+            raise ValueError(f"cot must be (6, {s.shape[1]}), got {c.shape}")
+        g = np.empty((7, s.shape[1]))
+        _check(lib().nbh_force_vjp(self._h, s.shape[1], _p(s), _p(c), _p(g)))
+        return g
+
     # ---- timing ----
     def set_profiling(self, on=True):
         _check(lib().nbh_set_profiling(self._h, int(bool(on))))
@@ -351,3 +367,44 @@ class Ensemble:
         """Slice of a concatenated (c, total) array belonging to system k."""
         o = self.offsets[k]
         return arr[:, o:o + self.counts[k]]
+
+
+# ----------------------------------------------------------------- adjoint --
+class Adjoint:
+    """nbh_adjoint: checkpointed reverse sweep of a fixed-dt Hermite run."""
+
+    def __init__(self, backend, n, every):
+        self.n = int(n)
+        self.backend = backend
+        h = C.c_void_p()
+        _check(lib().nbh_adj_create(backend._h, self.n, int(every), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nbh_adj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def forward(self, state0, dt, nsteps):
+        s = _state(state0)
+        out = np.empty((7, self.n))
+        _check(lib().nbh_adj_forward(self._h, _p(s), float(dt), int(nsteps), _p(out)))
+        return out
+
+    def backward(self, cot_final):
+        c = np.ascontiguousarray(cot_final, dtype=np.float64)
+        g = np.empty((7, self.n))
+        _check(lib().nbh_adj_backward(self._h, _p(c), _p(g)))
+        return g

@@ -11,6 +11,16 @@ for p in (os.path.join(ROOT, "oracle"), os.path.join(ROOT, "n-body-code_amd")):
         sys.path.insert(0, p)
 
 
+# Load the backend's HIP runtime before anything imports torch: torch bundles
+# its own libamdhip64.so.7 (same soname), and whichever loads first serves
+# the process.  Tests use torch on CPU tensors only.
+try:
+    import nbody_amd as _nbody_amd
+    _nbody_amd.lib()
+except Exception:  # not built: the tests that need it fail loudly themselves
+    pass
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run with -m gpu")
 
