@@ -46,8 +46,42 @@ def body_relerr(a, b):
     return np.sqrt(((a - b) ** 2).sum(0)) / np.sqrt((b ** 2).sum(0))
 
 
+def calibration_copy(nbytes=256 << 20):
+    """One device-to-device copy of a known size (a runtime blit kernel), so
+    that rocprofv3's FETCH_SIZE / WRITE_SIZE per dispatch can be scaled to
+    bytes: run under the profiler with --pmc-calibrate."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    try:
+        assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the pair kernels, as measured by rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE passes (scaled by the calibration copy) and
+    recorded in profiles/pmc_force_n1048576.md; None when not recorded."""
+    path = os.path.join(ROOT, "profiles", "pmc_force_n1048576.md")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch "):
+                _, kern, val = line.split()
+                out[kern] = float(val)
+    return out
+
+
 def run_precision(nb, prec, state, steps, warmup, log):
     with nb.Backend(prec, EPS * EPS) as b:
+        tune = b.autotune(N)            # launch shape for this size bucket, timed on the device
+        log(f"{prec}: autotuned shape {tune}")
         b.upload(state)                 # initial force, NBody::cycle
         f0 = b.download()[1]
         b.set_dt(DT)
@@ -74,6 +108,7 @@ def run_precision(nb, prec, state, steps, warmup, log):
         "kernel_tflops": FLOP_PER_PAIR * N * N / (kms * 1e-3) / 1e12,
         "energy_drift": abs(e1 - e0) / abs(e0),
         "shape": shape,
+        "autotune": tune,
     }
     log(f"{prec}: {res['ms_per_step']:.1f} ms/step, kernel {kms:.1f} ms, "
         f"{res['pairs_per_s']:.3e} pairs/s, drift {res['energy_drift']:.2e}")
@@ -173,6 +208,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--pmc-calibrate", action="store_true",
+                    help="start with a 256 MiB device-to-device copy (profiler calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -183,6 +220,8 @@ def main():
 
     import nbody_amd as nb
     log(f"device {nb.device_info()}")
+    if args.pmc_calibrate:
+        calibration_copy()
     t = time.perf_counter()
     state = nb.plummer(N, SEED)
     log(f"Plummer N={N} in {time.perf_counter() - t:.1f} s")
@@ -201,6 +240,7 @@ def main():
         cpu = cpu_baseline(state)
         log(f"cpu baseline {cpu['value']:.3e} pairs/s on {cpu['cores']} threads")
 
+    traffic = measured_traffic()
     out = {
         "metric": "pair-interactions/s at N=1M: mixed vs FP64 GPU @ median rel acc err <=1e-6",
         "value": rmx["pairs_per_s"],
@@ -225,11 +265,13 @@ def main():
         "energy_drift": {"fp64": r64["energy_drift"], "mixed": rmx["energy_drift"]},
         "launch_shape": rmx["shape"],
         "roofline": {"bound": "valu", "achieved": rmx["kernel_tflops"], "peak": PEAK["mixed"],
-                     "unit": "TFLOP/s", "frac": rmx["kernel_tflops"] / PEAK["mixed"], "traffic": None,
+                     "unit": "TFLOP/s", "frac": rmx["kernel_tflops"] / PEAK["mixed"],
+                     "traffic": traffic.get("k_force_mixed<256>"),
                      "kernel": "k_force_mixed<256>", "kernel_ms": rmx["kernel_ms"],
                      "flop_per_launch": FLOP_PER_PAIR * N * N},
         "roofline_fp64": {"bound": "valu", "achieved": r64["kernel_tflops"], "peak": PEAK["fp64"],
-                          "unit": "TFLOP/s", "frac": r64["kernel_tflops"] / PEAK["fp64"], "traffic": None,
+                          "unit": "TFLOP/s", "frac": r64["kernel_tflops"] / PEAK["fp64"],
+                          "traffic": traffic.get("k_force_f64<256>"),
                           "kernel": "k_force_f64<256>", "kernel_ms": r64["kernel_ms"],
                           "flop_per_launch": FLOP_PER_PAIR * N * N},
         "cpu_baseline": cpu,

@@ -51,6 +51,7 @@ typedef struct nbh_config {
     int block;      /* i-bodies (= threads) per workgroup and j-tile length:
                        64, 128 or 256; 0 = pick by size */
     int jsplit;     /* j-range split across workgroups; 0 = pick by size */
+    int ipl;        /* targets per lane, 1 or 2; 0 = pick by size */
 } nbh_config;
 
 /* Description of the last failure on this thread ("" if none). */
@@ -152,6 +153,18 @@ int nbh_reset_timing(nbh_ctx *ctx);
 /* Launch shape actually used for n bodies: block, jsplit, workgroups. */
 int nbh_launch_shape(nbh_ctx *ctx, int n, int *block, int *jsplit,
                      int *workgroups);
+/* Targets per lane of that shape. */
+int nbh_launch_ipl(nbh_ctx *ctx, int n, int *ipl);
+
+/* Autotuner of the single-system force: times block {64,128,256} x targets
+ * per lane {1,2} x j-split {1..32} for n bodies on this context's device and
+ * precision (a Plummer sphere, on a source range of max(min(n, 65536), n/16)
+ * bodies so that a trial is short) and keeps the fastest for n's size
+ * bucket (floor(log2 n)) and precision.  Later evaluations in that bucket use
+ * it unless the config forces block/jsplit/ipl.  Outputs may be NULL; ms is
+ * the winner's trial time. */
+int nbh_autotune(nbh_ctx *ctx, int n, int *block, int *ipl, int *jsplit,
+                 double *ms);
 
 /* Device properties: compute units and wave size. */
 int nbh_device_info(int device, int *compute_units, int *wave_size,

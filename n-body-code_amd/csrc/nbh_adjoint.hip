@@ -276,6 +276,7 @@ int adj_force(nbh_adjoint* a, double* st, double* f)
     const int n = a->n;
     nbh::Shape sh{};
     sh.block = 64;
+    sh.ipl = 1;
     const int tiles = (n + 63) / 64;
     const int split = nbh::adj_force_splits(n, c->cus);
     const int per = (tiles + split - 1) / split;

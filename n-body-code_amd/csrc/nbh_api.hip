@@ -7,6 +7,7 @@
 // every force_calculation (src/kernel.cpp:54-138, :397).
 #include "nbh_ctx.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -93,26 +94,47 @@ using nbh::state_of;
 
 namespace {
 
-nbh::Shape pick_shape(const nbh_ctx* c, int n)
+int size_bucket(int n)
+{
+    int b = 0;
+    while ((2 << b) <= n && b < 31) ++b;
+    return b;
+}
+
+// Shape with the given block / targets-per-lane / requested split; the split
+// is clamped to the source tiles and rounded to whole tiles per split.
+nbh::Shape make_shape(int n, int block, int ipl, int split, int nj = 0)
 {
     nbh::Shape s{};
-    int block = c->cfg.block;
-    if (block == 0) block = n >= 131072 ? 256 : (n >= 16384 ? 128 : 64);
     s.block = block;
-    const int iblocks = (n + block - 1) / block;
-    const int jtiles = (n + block - 1) / block;
-    int split = c->cfg.jsplit;
-    if (split <= 0) {
-        // aim for about 8 workgroups per CU
-        const int want = 8 * (c->cus > 0 ? c->cus : 256);
-        split = (want + iblocks - 1) / iblocks;
-    }
-    if (split > jtiles) split = jtiles;
-    if (split < 1) split = 1;
+    s.ipl = ipl;
+    const int src = nj > 0 ? nj : n;
+    const int jtiles = (src + block - 1) / block;
+    split = std::max(1, std::min(split, jtiles));
     const int tiles_per = (jtiles + split - 1) / split;
     s.jchunk = tiles_per * block;
-    s.jsplit = (n + s.jchunk - 1) / s.jchunk;
+    s.jsplit = (src + s.jchunk - 1) / s.jchunk;
     return s;
+}
+
+nbh::Shape pick_shape(const nbh_ctx* c, int n)
+{
+    const nbh_config& f = c->cfg;
+    if (f.block == 0 && f.jsplit == 0 && f.ipl == 0) {
+        const auto& t = c->tuned[f.precision][size_bucket(n)];
+        if (t.block) return make_shape(n, t.block, t.ipl, t.jsplit);
+    }
+    const int block = f.block ? f.block : (n >= 131072 ? 256 : (n >= 16384 ? 128 : 64));
+    const int ipl = f.ipl ? f.ipl : 1;
+    int split = f.jsplit;
+    if (split <= 0) {
+        // enough workgroups that the last partial round of resident
+        // workgroups is a small fraction of the launch (~128 per CU)
+        const int iblocks = (n + block * ipl - 1) / (block * ipl);
+        const int want = 128 * (c->cus > 0 ? c->cus : 256);
+        split = std::min(64, (want + iblocks - 1) / iblocks);
+    }
+    return make_shape(n, block, ipl, split);
 }
 
 int check_cfg(const nbh_config* cfg)
@@ -125,6 +147,7 @@ int check_cfg(const nbh_config* cfg)
     if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256)
         return fail(NBH_ERR_ARG, "block must be 0, 64, 128 or 256, got %d", cfg->block);
     if (cfg->jsplit < 0) return fail(NBH_ERR_ARG, "jsplit must be >= 0, got %d", cfg->jsplit);
+    if (cfg->ipl < 0 || cfg->ipl > 2) return fail(NBH_ERR_ARG, "ipl must be 0, 1 or 2, got %d", cfg->ipl);
     return NBH_OK;
 }
 
@@ -258,7 +281,90 @@ int nbh_launch_shape(nbh_ctx* c, int n, int* block, int* jsplit, int* workgroups
     const nbh::Shape s = pick_shape(c, n);
     if (block) *block = s.block;
     if (jsplit) *jsplit = s.jsplit;
-    if (workgroups) *workgroups = ((n + s.block - 1) / s.block) * s.jsplit;
+    if (workgroups) *workgroups = ((n + s.block * s.ipl - 1) / (s.block * s.ipl)) * s.jsplit;
+    return NBH_OK;
+}
+
+int nbh_launch_ipl(nbh_ctx* c, int n, int* ipl)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    if (ipl) *ipl = pick_shape(c, n).ipl;
+    return NBH_OK;
+}
+
+int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_out, double* ms_out)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    const size_t nb = sizeof(double) * (size_t)n;
+    std::vector<double> h((size_t)7 * n);
+    double* p = h.data();
+    if (nbh_plummer(n, 1, p, p + n, p + 2 * (size_t)n, p + 3 * (size_t)n, p + 4 * (size_t)n, p + 5 * (size_t)n,
+                    p + 6 * (size_t)n) != NBH_OK)
+        return fail(NBH_ERR_ARG, "autotune: cannot build a test system of %d bodies", n);
+    if (c->in.reserve(7 * nb) != hipSuccess || c->out.reserve(6 * nb) != hipSuccess)
+        return fail(NBH_ERR_ALLOC, "autotune: device allocation failed");
+    NBH_HIP(hipMemcpyAsync(c->in.p, p, 7 * nb, hipMemcpyHostToDevice, c->stream));
+    const nbh::State st = state_of(c->in.d(), n);
+    const nbh::Forces fo = forces_of(c->out.d(), n);
+    const int nj = n <= 65536 ? n : std::max(65536, n / 16);
+    hipEvent_t e0, e1;
+    NBH_HIP(hipEventCreate(&e0));
+    NBH_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int bb = 0, bi = 0, bs = 0;
+    int rc = NBH_OK;
+    for (int block : {64, 128, 256}) {
+        if (n >= 65536 && block == 64) continue;
+        for (int ipl : {1, 2}) {
+            int last = -1;
+            for (int split : {1, 2, 4, 8, 16, 32}) {
+                const nbh::Shape sh = make_shape(n, block, ipl, split, nj);
+                if (sh.jsplit == last) continue;
+                last = sh.jsplit;
+                if (sh.jsplit > 1 && c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n) != hipSuccess) {
+                    rc = fail(NBH_ERR_ALLOC, "autotune: partial-sum allocation failed");
+                    break;
+                }
+                double mn = 1e300;
+                for (int rep = 0; rep < 2 && rc == NBH_OK; ++rep) {
+                    hipError_t e = hipEventRecord(e0, c->stream);
+                    if (e == hipSuccess)
+                        e = nbh::launch_force(c->cfg.precision, st, n, c->cfg.eps2, sh, fo, c->partial.d(), c->stream,
+                                              nullptr, nullptr, nj);
+                    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+                    if (e == hipSuccess) e = hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                    if (e != hipSuccess) rc = fail(NBH_ERR_HIP, "autotune trial failed: %s", hipGetErrorString(e));
+                    if (rep > 0) mn = ms;  // the first launch warms up
+                }
+                if (rc) break;
+                if (mn < best) {
+                    best = mn;
+                    bb = block;
+                    bi = ipl;
+                    bs = split;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    auto& t = c->tuned[c->cfg.precision][size_bucket(n)];
+    t.block = bb;
+    t.ipl = bi;
+    t.jsplit = bs;
+    drop_graph(c);
+    if (block_out) *block_out = bb;
+    if (ipl_out) *ipl_out = bi;
+    if (jsplit_out) *jsplit_out = pick_shape(c, n).jsplit;
+    if (ms_out) *ms_out = best;
     return NBH_OK;
 }
 

@@ -80,6 +80,11 @@ struct nbh_ctx {
     int g_n = -1;
     bool g_adaptive = false;
 
+    // autotuned launch shapes, [precision][floor(log2 n)]: block, ipl, jsplit
+    struct Tuned {
+        int block = 0, ipl = 0, jsplit = 0;
+    } tuned[2][32];
+
     // profiling
     bool profiling = false;
     std::vector<nbh::EventPair> ev_free, ev_pending;

@@ -27,19 +27,21 @@ struct Forces {
 
 // One force evaluation's launch shape.
 struct Shape {
-    int block;   // threads per workgroup = i-bodies per workgroup = j-tile length
+    int block;   // lanes per workgroup = j-tile length (64, 128 or 256)
+    int ipl;     // targets per lane (1 or 2): block*ipl targets per workgroup
     int jsplit;  // workgroups along j; partial sums reduced in a second pass
     int jchunk;  // j-bodies per split (multiple of block)
 };
 
-// Pair-kernel launch: writes forces for all n targets from all n sources.
-// When shape.jsplit > 1, `partial` must hold 6*jsplit*n doubles and the
-// split-reduction kernel is launched after the pair kernel.  ev0/ev1 (may be
-// null) are recorded around the pair kernel only.
+// Pair-kernel launch: writes forces for all n targets from all n sources
+// (from the first nj sources when 0 < nj < n: the autotuner's short trial
+// launches).  When shape.jsplit > 1, `partial` must hold 6*jsplit*n doubles
+// and the split-reduction kernel is launched after the pair kernel.  ev0/ev1
+// (may be null) are recorded around the pair kernel only.
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out,
                         double* partial, hipStream_t st, hipEvent_t ev0,
-                        hipEvent_t ev1);
+                        hipEvent_t ev1, int nj = 0);
 
 // Hermite predictor in place (x, v) from forces f0 with dt read from *dt.
 hipError_t launch_predict(int n, const double* dt, double* x, double* y,

@@ -32,27 +32,39 @@ constexpr int kOBlock = 256;  // threads per workgroup of the O(n) kernels
 
 struct PairArgs {
     State s;
-    int n;
-    int jchunk;
+    int n;                // targets: bodies 0..n-1
+    int nj;               // sources: bodies 0..nj-1 (n except in the autotuner's trial launches)
+    int jchunk;           // sources per j-split
     double eps2;
     Forces out;
     size_t split_stride;  // element offset between j-split partial planes
 };
 
-template <int BLOCK, bool MASK>
+// BLOCK lanes per workgroup, IPL targets per lane (lane t holds targets
+// base + t + q*BLOCK, q < IPL), so each broadcast source record read from
+// LDS feeds IPL pair evaluations.
+template <int BLOCK, int IPL, bool MASK>
 __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
 {
+    constexpr int UNROLL = IPL == 1 ? 4 : 2;
     __shared__ JB64 tile[2][BLOCK];
     const int t = threadIdx.x;
-    const int i = blockIdx.x * BLOCK + t;
-    const int ic = i < a.n ? i : a.n - 1;
-    const double xi = a.s.x[ic], yi = a.s.y[ic], zi = a.s.z[ic];
-    const double vxi = a.s.vx[ic], vyi = a.s.vy[ic], vzi = a.s.vz[ic];
+    const int base = blockIdx.x * BLOCK * IPL + t;
     const double eps2 = a.eps2;
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double xi[IPL], yi[IPL], zi[IPL], vxi[IPL], vyi[IPL], vzi[IPL];
+    double acc[IPL][6];
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int i = base + q * BLOCK;
+        const int ic = i < a.n ? i : a.n - 1;
+        xi[q] = a.s.x[ic]; yi[q] = a.s.y[ic]; zi[q] = a.s.z[ic];
+        vxi[q] = a.s.vx[ic]; vyi[q] = a.s.vy[ic]; vzi[q] = a.s.vz[ic];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[q][c] = 0.0;
+    }
 
     const int jbeg = blockIdx.y * a.jchunk;
-    const int jend = min(a.n, jbeg + a.jchunk);
+    const int jend = min(a.nj, jbeg + a.jchunk);
     JB64 st{};
     if (jbeg + t < jend) st = load_jb64(a.s, jbeg + t);
     tile[0][t] = st;
@@ -64,40 +76,62 @@ __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
         if (more && jt + BLOCK + t < jend) st = load_jb64(a.s, jt + BLOCK + t);
         const JB64* tb = tile[buf];
         if (cnt == BLOCK) {
-#pragma unroll 4
-            for (int k = 0; k < BLOCK; ++k)
-                pair_f64<MASK>(tb[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+#pragma unroll UNROLL
+            for (int k = 0; k < BLOCK; ++k) {
+                const JB64 b = tb[k];
+#pragma unroll
+                for (int q = 0; q < IPL; ++q)
+                    pair_f64<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
+                                   eps2, acc[q]);
+            }
         } else {
-            for (int k = 0; k < cnt; ++k)
-                pair_f64<MASK>(tb[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+            for (int k = 0; k < cnt; ++k) {
+                const JB64 b = tb[k];
+#pragma unroll
+                for (int q = 0; q < IPL; ++q)
+                    pair_f64<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
+                                   eps2, acc[q]);
+            }
         }
         if (more) tile[buf ^ 1][t] = st;
         __syncthreads();
         buf ^= 1;
     }
-    if (i < a.n) {
-        const size_t o = (size_t)blockIdx.y * a.split_stride + i;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) a.out.c[c][o] = acc[c];
+    for (int q = 0; q < IPL; ++q) {
+        const int i = base + q * BLOCK;
+        if (i < a.n) {
+            const size_t o = (size_t)blockIdx.y * a.split_stride + i;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a.out.c[c][o] = acc[q][c];
+        }
     }
 }
 
-template <int BLOCK, bool MASK>
+template <int BLOCK, int IPL, bool MASK>
 __global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a)
 {
     static_assert(BLOCK % kFold == 0, "tile must hold whole fold runs");
+    constexpr int UNROLL = IPL == 1 ? 4 : 2;
     __shared__ JBMX tile[2][BLOCK];
     const int t = threadIdx.x;
-    const int i = blockIdx.x * BLOCK + t;
-    const int ic = i < a.n ? i : a.n - 1;
-    const double xi = a.s.x[ic], yi = a.s.y[ic], zi = a.s.z[ic];
-    const float vxi = (float)a.s.vx[ic], vyi = (float)a.s.vy[ic], vzi = (float)a.s.vz[ic];
+    const int base = blockIdx.x * BLOCK * IPL + t;
     const float eps2 = (float)a.eps2;
-    float hi[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    float lo[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    double xi[IPL], yi[IPL], zi[IPL];
+    float vxi[IPL], vyi[IPL], vzi[IPL];
+    float hi[IPL][6], lo[IPL][6];
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int i = base + q * BLOCK;
+        const int ic = i < a.n ? i : a.n - 1;
+        xi[q] = a.s.x[ic]; yi[q] = a.s.y[ic]; zi[q] = a.s.z[ic];
+        vxi[q] = (float)a.s.vx[ic]; vyi[q] = (float)a.s.vy[ic]; vzi[q] = (float)a.s.vz[ic];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) hi[q][c] = lo[q][c] = 0.f;
+    }
 
     const int jbeg = blockIdx.y * a.jchunk;
-    const int jend = min(a.n, jbeg + a.jchunk);
+    const int jend = min(a.nj, jbeg + a.jchunk);
     JBMX st{};
     if (jbeg + t < jend) st = load_jbmx(a.s, jbeg + t);
     tile[0][t] = st;
@@ -109,26 +143,46 @@ __global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a)
         if (more && jt + BLOCK + t < jend) st = load_jbmx(a.s, jt + BLOCK + t);
         const JBMX* tb = tile[buf];
         for (int k0 = 0; k0 < cnt; k0 += kFold) {
-            float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            float acc[IPL][6];
+#pragma unroll
+            for (int q = 0; q < IPL; ++q)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[q][c] = 0.f;
             if (k0 + kFold <= cnt) {
-#pragma unroll 4
-                for (int k = 0; k < kFold; ++k)
-                    pair_mx<MASK>(tb[k0 + k], jt + k0 + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+#pragma unroll UNROLL
+                for (int k = 0; k < kFold; ++k) {
+                    const JBMX b = tb[k0 + k];
+#pragma unroll
+                    for (int q = 0; q < IPL; ++q)
+                        pair_mx<MASK>(b, jt + k0 + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q],
+                                      vzi[q], eps2, acc[q]);
+                }
             } else {
-                for (int k = k0; k < cnt; ++k)
-                    pair_mx<MASK>(tb[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+                for (int k = k0; k < cnt; ++k) {
+                    const JBMX b = tb[k];
+#pragma unroll
+                    for (int q = 0; q < IPL; ++q)
+                        pair_mx<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
+                                      eps2, acc[q]);
+                }
             }
 #pragma unroll
-            for (int c = 0; c < 6; ++c) two_sum_fold(hi[c], lo[c], acc[c]);
+            for (int q = 0; q < IPL; ++q)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) two_sum_fold(hi[q][c], lo[q][c], acc[q][c]);
         }
         if (more) tile[buf ^ 1][t] = st;
         __syncthreads();
         buf ^= 1;
     }
-    if (i < a.n) {
-        const size_t o = (size_t)blockIdx.y * a.split_stride + i;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) a.out.c[c][o] = (double)hi[c] + (double)lo[c];
+    for (int q = 0; q < IPL; ++q) {
+        const int i = base + q * BLOCK;
+        if (i < a.n) {
+            const size_t o = (size_t)blockIdx.y * a.split_stride + i;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a.out.c[c][o] = (double)hi[q][c] + (double)lo[q][c];
+        }
     }
 }
 
@@ -386,20 +440,20 @@ __global__ __launch_bounds__(kOBlock) void k_check_finite(int n, Forces f, int* 
 
 inline int oblocks(int n) { return (n + kOBlock - 1) / kOBlock; }
 
-template <int BLOCK>
+template <int BLOCK, int IPL>
 hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, hipStream_t st)
 {
-    const dim3 grid((a.n + BLOCK - 1) / BLOCK, jsplit);
+    const dim3 grid((a.n + BLOCK * IPL - 1) / (BLOCK * IPL), jsplit);
     if (precision == 0) {
         if (mask)
-            k_force_f64<BLOCK, true><<<grid, BLOCK, 0, st>>>(a);
+            k_force_f64<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a);
         else
-            k_force_f64<BLOCK, false><<<grid, BLOCK, 0, st>>>(a);
+            k_force_f64<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a);
     } else {
         if (mask)
-            k_force_mixed<BLOCK, true><<<grid, BLOCK, 0, st>>>(a);
+            k_force_mixed<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a);
         else
-            k_force_mixed<BLOCK, false><<<grid, BLOCK, 0, st>>>(a);
+            k_force_mixed<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a);
     }
     return hipGetLastError();
 }
@@ -408,11 +462,12 @@ hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, 
 
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out, double* partial,
-                        hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+                        hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int nj)
 {
     PairArgs a;
     a.s = s;
     a.n = n;
+    a.nj = nj > 0 ? nj : n;
     a.jchunk = shape.jchunk;
     a.eps2 = eps2;
     if (shape.jsplit > 1) {
@@ -426,10 +481,14 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
     const bool mask = (eps2 == 0.0);
     hipError_t e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
-    switch (shape.block) {
-    case 64: e = launch_pair<64>(precision, mask, a, shape.jsplit, st); break;
-    case 128: e = launch_pair<128>(precision, mask, a, shape.jsplit, st); break;
-    case 256: e = launch_pair<256>(precision, mask, a, shape.jsplit, st); break;
+    const int key = shape.block * 10 + shape.ipl;
+    switch (key) {
+    case 641: e = launch_pair<64, 1>(precision, mask, a, shape.jsplit, st); break;
+    case 642: e = launch_pair<64, 2>(precision, mask, a, shape.jsplit, st); break;
+    case 1281: e = launch_pair<128, 1>(precision, mask, a, shape.jsplit, st); break;
+    case 1282: e = launch_pair<128, 2>(precision, mask, a, shape.jsplit, st); break;
+    case 2561: e = launch_pair<256, 1>(precision, mask, a, shape.jsplit, st); break;
+    case 2562: e = launch_pair<256, 2>(precision, mask, a, shape.jsplit, st); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;

@@ -39,7 +39,7 @@ class NBodyError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("eps2", C.c_double),
-                ("block", C.c_int), ("jsplit", C.c_int)]
+                ("block", C.c_int), ("jsplit", C.c_int), ("ipl", C.c_int)]
 
 
 _lib = None
@@ -73,6 +73,9 @@ def lib():
     L.nbh_timing.argtypes = [vp, _D, C.POINTER(C.c_int64), _D, C.POINTER(C.c_int64)]
     L.nbh_reset_timing.argtypes = [vp]
     L.nbh_launch_shape.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.nbh_launch_ipl.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.nbh_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                               C.POINTER(C.c_double)]
     L.nbh_device_info.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
     L.nbh_plummer.argtypes = [C.c_int, C.c_uint64] + [_D] * 7
     _I = C.POINTER(C.c_int)
@@ -140,10 +143,10 @@ def device_info(device=0):
 class Backend:
     """One nbh_ctx: a device, a stream, buffers, a resident state."""
 
-    def __init__(self, precision="fp64", eps2=1e-7, device=0, block=0, jsplit=0):
+    def __init__(self, precision="fp64", eps2=1e-7, device=0, block=0, jsplit=0, ipl=0):
         L = lib()
         self._cfg = Config(device, PRECISIONS[precision] if isinstance(precision, str) else int(precision),
-                           float(eps2), int(block), int(jsplit))
+                           float(eps2), int(block), int(jsplit), int(ipl))
         h = C.c_void_p()
         _check(L.nbh_create(C.byref(self._cfg), C.byref(h)))
         self._h = h
@@ -166,8 +169,10 @@ class Backend:
     def __exit__(self, *exc):
         self.close()
 
-    def configure(self, precision=None, eps2=None, block=None, jsplit=None):
+    def configure(self, precision=None, eps2=None, block=None, jsplit=None, ipl=None):
         c = self._cfg
+        if ipl is not None:
+            c.ipl = int(ipl)
         if precision is not None:
             c.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         if eps2 is not None:
@@ -189,9 +194,16 @@ class Backend:
         return (out, ms.value) if return_ms else out
 
     def launch_shape(self, n):
-        b, s, w = C.c_int(), C.c_int(), C.c_int()
+        b, s, w, q = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _check(lib().nbh_launch_shape(self._h, int(n), C.byref(b), C.byref(s), C.byref(w)))
-        return {"block": b.value, "jsplit": s.value, "workgroups": w.value}
+        _check(lib().nbh_launch_ipl(self._h, int(n), C.byref(q)))
+        return {"block": b.value, "ipl": q.value, "jsplit": s.value, "workgroups": w.value}
+
+    def autotune(self, n):
+        """Times launch shapes for n bodies and keeps the fastest for n's size bucket."""
+        b, q, s, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        _check(lib().nbh_autotune(self._h, int(n), C.byref(b), C.byref(q), C.byref(s), C.byref(ms)))
+        return {"block": b.value, "ipl": q.value, "jsplit": s.value, "trial_ms": ms.value}
 
     # ---- resident integrator ----
     def upload(self, state):

@@ -72,14 +72,40 @@ def test_fp64_ragged_sizes(be64, oracle, nb, n):
 
 @pytest.mark.parametrize("block", [64, 128, 256])
 @pytest.mark.parametrize("jsplit", [1, 2, 7, 0])
-def test_fp64_launch_shapes_agree(be64, oracle, nb, block, jsplit):
-    _banner(f"shape block={block} jsplit={jsplit}")
+@pytest.mark.parametrize("ipl", [1, 2])
+def test_fp64_launch_shapes_agree(be64, oracle, nb, block, jsplit, ipl):
+    _banner(f"shape block={block} jsplit={jsplit} ipl={ipl}")
     s = nb.plummer(3000, 77)
-    be64.configure(eps2=1e-3, block=block, jsplit=jsplit)
+    be64.configure(eps2=1e-3, block=block, jsplit=jsplit, ipl=ipl)
     try:
         check64(be64.force(s), oracle.force(s, 1e-3))
     finally:
-        be64.configure(block=0, jsplit=0)
+        be64.configure(block=0, jsplit=0, ipl=0)
+
+
+@pytest.mark.parametrize("ipl", [1, 2])
+def test_mixed_targets_per_lane_identical(bemx, nb, ipl):
+    """Each target sums its sources in the same order whatever the number of
+    targets per lane, so IPL cannot change the mixed result."""
+    s = nb.plummer(3001, 78)
+    bemx.configure(eps2=1e-3, block=128, jsplit=3, ipl=1)
+    ref = bemx.force(s)
+    bemx.configure(ipl=ipl)
+    try:
+        assert np.array_equal(bemx.force(s), ref)
+    finally:
+        bemx.configure(block=0, jsplit=0, ipl=0)
+
+
+def test_autotune_keeps_parity(be64, oracle, nb):
+    s = nb.plummer(2500, 79)
+    be64.configure(eps2=1e-3, block=0, jsplit=0, ipl=0)
+    t = be64.autotune(2500)
+    _banner(f"autotune n=2500: {t}")
+    assert t["block"] in (64, 128, 256) and t["ipl"] in (1, 2)
+    shape = be64.launch_shape(2500)
+    assert (shape["block"], shape["ipl"]) == (t["block"], t["ipl"])
+    check64(be64.force(s), oracle.force(s, 1e-3))
 
 
 def test_fp64_deterministic(be64, nb):
