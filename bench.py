@@ -78,6 +78,12 @@ def measured_traffic():
     return out
 
 
+def kname(prec, res):
+    """Pair-kernel instance of a run: k_force_{f64,mixed}<block,targets per lane>."""
+    sh = res["shape"]
+    return f"k_force_{'f64' if prec == 'fp64' else 'mixed'}<{sh['block']},{sh['ipl']}>"
+
+
 def run_precision(nb, prec, state, steps, warmup, log):
     with nb.Backend(prec, EPS * EPS) as b:
         tune = b.autotune(N)            # launch shape for this size bucket, timed on the device
@@ -266,13 +272,13 @@ def main():
         "launch_shape": rmx["shape"],
         "roofline": {"bound": "valu", "achieved": rmx["kernel_tflops"], "peak": PEAK["mixed"],
                      "unit": "TFLOP/s", "frac": rmx["kernel_tflops"] / PEAK["mixed"],
-                     "traffic": traffic.get("k_force_mixed<256>"),
-                     "kernel": "k_force_mixed<256>", "kernel_ms": rmx["kernel_ms"],
+                     "traffic": traffic.get(kname("mixed", rmx)),
+                     "kernel": kname("mixed", rmx), "kernel_ms": rmx["kernel_ms"],
                      "flop_per_launch": FLOP_PER_PAIR * N * N},
         "roofline_fp64": {"bound": "valu", "achieved": r64["kernel_tflops"], "peak": PEAK["fp64"],
                           "unit": "TFLOP/s", "frac": r64["kernel_tflops"] / PEAK["fp64"],
-                          "traffic": traffic.get("k_force_f64<256>"),
-                          "kernel": "k_force_f64<256>", "kernel_ms": r64["kernel_ms"],
+                          "traffic": traffic.get(kname("fp64", r64)),
+                          "kernel": kname("fp64", r64), "kernel_ms": r64["kernel_ms"],
                           "flop_per_launch": FLOP_PER_PAIR * N * N},
         "cpu_baseline": cpu,
         "ensemble": ens,
