@@ -103,6 +103,11 @@ def run_precision(nb, prec, state, steps, warmup, log):
         tm = b.timing()
         e1 = sum(b.energy())
         shape = b.launch_shape(N)
+        # drop-in boundary: host arrays in and out (PCIe included), as
+        # force_calculation is called
+        t2 = time.perf_counter()
+        _, dropin_kernel_ms = b.force(state, return_ms=True)
+        dropin_wall = time.perf_counter() - t2
     sec = t1 - t0
     kms = tm["force_ms"] / max(tm["force_launches"], 1)
     res = {
@@ -115,6 +120,8 @@ def run_precision(nb, prec, state, steps, warmup, log):
         "energy_drift": abs(e1 - e0) / abs(e0),
         "shape": shape,
         "autotune": tune,
+        "dropin": {"wall_ms": 1e3 * dropin_wall, "kernel_ms": dropin_kernel_ms,
+                   "pairs_per_s_pcie_inclusive": N * N / dropin_wall},
     }
     log(f"{prec}: {res['ms_per_step']:.1f} ms/step, kernel {kms:.1f} ms, "
         f"{res['pairs_per_s']:.3e} pairs/s, drift {res['energy_drift']:.2e}")
@@ -270,6 +277,7 @@ def main():
         "steps_per_s": {"fp64": r64["steps_per_s"], "mixed": rmx["steps_per_s"]},
         "energy_drift": {"fp64": r64["energy_drift"], "mixed": rmx["energy_drift"]},
         "launch_shape": rmx["shape"],
+        "dropin_host_arrays": {"fp64": r64["dropin"], "mixed": rmx["dropin"]},
         "roofline": {"bound": "valu", "achieved": rmx["kernel_tflops"], "peak": PEAK["mixed"],
                      "unit": "TFLOP/s", "frac": rmx["kernel_tflops"] / PEAK["mixed"],
                      "traffic": traffic.get(kname("mixed", rmx)),
