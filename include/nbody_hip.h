@@ -263,6 +263,21 @@ int nbh_reference_ic(int n, double vmax, double rmax, double mmin,
                      double mmax, double *x, double *y, double *z, double *m,
                      double *vx, double *vy, double *vz);
 
+/* ---- body-state files (host code) ------------------------------------- */
+/* No counterpart in the reference (it has no state I/O).  Little-endian
+ * binary: a 64-byte header -- char magic[8] = "NBHSTATE", uint32 version = 1,
+ * uint32 reserved = 0, int64 n, double time, int64 step, 24 zero bytes --
+ * then x, y, z, m, vx, vy, vz as n doubles each. */
+int nbh_state_save(const char *path, int n, const double *x, const double *y,
+                   const double *z, const double *m, const double *vx,
+                   const double *vy, const double *vz, double time,
+                   int64_t step);
+/* Reads the header: body count, time and step of the file. */
+int nbh_state_info(const char *path, int *n, double *time, int64_t *step);
+/* Reads the bodies; n must equal the file's body count. */
+int nbh_state_load(const char *path, int n, double *x, double *y, double *z,
+                   double *m, double *vx, double *vy, double *vz);
+
 #ifdef __cplusplus
 }
 #endif

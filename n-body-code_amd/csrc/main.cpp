@@ -19,7 +19,7 @@ const char* kUsage =
     "Usage: ./nbody_app [--N <value>] [--Ncycle <value>]\n"
     "  extensions: [--eps <softening>] [--dt <fixed step>] [--ic ref|plummer] [--seed <int>]\n"
     "              [--precision fp64|mixed] [--resident] [--graph] [--energy compat|physical]\n"
-    "              [--device <ordinal>]";
+    "              [--device <ordinal>] [--load <state file>] [--save <state file>]";
 
 int positive_int(const std::string& name, const char* text)
 {
@@ -39,7 +39,8 @@ int main(int argc, char* argv[])
     int N = 16384, Ncycle = 10;
     ForceBackend fb;
     double fixed_dt = 0.0;
-    std::string ic = "ref", energy = "compat";
+    std::string ic = "ref", energy = "compat", load_path, save_path;
+    bool n_given = false;
     uint64_t seed = 1;
     bool resident = false, graph = false;
 
@@ -49,6 +50,7 @@ int main(int argc, char* argv[])
         try {
             if (arg == "--N" && has) {
                 N = positive_int("N", argv[++i]);
+                n_given = true;
             } else if (arg == "--Ncycle" && has) {
                 Ncycle = positive_int("Ncycle", argv[++i]);
             } else if (arg == "--eps" && has) {
@@ -74,6 +76,10 @@ int main(int argc, char* argv[])
                     throw std::invalid_argument("energy must be compat or physical.");
             } else if (arg == "--device" && has) {
                 fb.device = std::stoi(argv[++i]);
+            } else if (arg == "--load" && has) {
+                load_path = argv[++i];
+            } else if (arg == "--save" && has) {
+                save_path = argv[++i];
             } else if (arg == "--resident") {
                 resident = true;
             } else if (arg == "--graph") {
@@ -89,6 +95,18 @@ int main(int argc, char* argv[])
         }
     }
 
+    if (!load_path.empty()) {
+        int nf = 0;
+        if (nbh_state_info(load_path.c_str(), &nf, nullptr, nullptr) != NBH_OK) {
+            std::cerr << "Error reading --load: " << nbh_last_error() << std::endl;
+            return 1;
+        }
+        if (n_given && nf != N) {
+            std::cerr << "Error: --N " << N << " but " << load_path << " holds " << nf << " bodies" << std::endl;
+            return 1;
+        }
+        N = nf;
+    }
     std::cout << "Using N = " << N << ", Ncycle = " << Ncycle << std::endl;
     force_backend_configure(fb);
 
@@ -100,8 +118,15 @@ int main(int argc, char* argv[])
     nbody.compat_energy = energy == "compat";
     nbody.resident = resident;
     nbody.use_graph = graph;
+    nbody.load_path = load_path;
     nbody.cycle(Ncycle);
     const double total = wall() - start;
+    if (!save_path.empty() &&
+        nbh_state_save(save_path.c_str(), N, nbody.h_par_x, nbody.h_par_y, nbody.h_par_z, nbody.h_par_w,
+                       nbody.h_vel_x, nbody.h_vel_y, nbody.h_vel_z, nbody.sim_time, Ncycle) != NBH_OK) {
+        std::cerr << "Error writing --save: " << nbh_last_error() << std::endl;
+        return 1;
+    }
 
     std::cout << std::fixed << std::setprecision(9);
     std::cout << "Time stamp(start): " << start << std::endl << std::endl;

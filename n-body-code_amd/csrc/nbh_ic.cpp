@@ -9,9 +9,14 @@
 #include "../../include/nbody_hip.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <random>
 #include <vector>
+
+namespace nbh {
+int fail(int code, const char* fmt, ...);  // nbh_api.hip: records the last error
+}
 
 namespace {
 
@@ -70,7 +75,8 @@ extern "C" {
 int nbh_plummer(int n, uint64_t seed, double* x, double* y, double* z, double* m, double* vx,
                 double* vy, double* vz)
 {
-    if (n < 1 || !x || !y || !z || !m || !vx || !vy || !vz) return NBH_ERR_ARG;
+    if (n < 1 || !x || !y || !z || !m || !vx || !vy || !vz)
+        return nbh::fail(NBH_ERR_ARG, "nbh_plummer: need n >= 1 and seven arrays (n=%d)", n);
     std::mt19937_64 g(seed);
     // Aarseth, Henon & Wielen (1974) with scale length 1, then the
     // Henon-unit scaling a = 3 pi / 16 (E = -1/4 for the untruncated model).
@@ -113,7 +119,8 @@ int nbh_plummer(int n, uint64_t seed, double* x, double* y, double* z, double* m
 int nbh_reference_ic(int n, double vmax, double rmax, double mmin, double mmax, double* x,
                      double* y, double* z, double* m, double* vx, double* vy, double* vz)
 {
-    if (n < 1 || !x || !y || !z || !m || !vx || !vy || !vz) return NBH_ERR_ARG;
+    if (n < 1 || !x || !y || !z || !m || !vx || !vy || !vz)
+        return nbh::fail(NBH_ERR_ARG, "nbh_reference_ic: need n >= 1 and seven arrays (n=%d)", n);
     std::mt19937 engine(42);
     std::uniform_real_distribution<double> dist(-1.0, 1.0);
     const double alpha = -2.35 + 1.0;  // Salpeter index + 1 (include/nbody.h:28)
@@ -150,6 +157,89 @@ int nbh_reference_ic(int n, double vmax, double rmax, double mmin, double mmax, 
         vz[i] = vz[i] * f;
     }
     return NBH_OK;
+}
+
+namespace {
+
+struct StateHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t reserved;
+    int64_t n;
+    double time;
+    int64_t step;
+    unsigned char pad[24];
+};
+static_assert(sizeof(StateHeader) == 64, "state header is 64 bytes");
+
+const char kMagic[8] = {'N', 'B', 'H', 'S', 'T', 'A', 'T', 'E'};
+
+int read_header(FILE* f, const char* path, StateHeader* h)
+{
+    if (std::fread(h, sizeof *h, 1, f) != 1)
+        return nbh::fail(NBH_ERR_ARG, "%s: too short for a state header", path);
+    if (std::memcmp(h->magic, kMagic, 8) != 0) return nbh::fail(NBH_ERR_ARG, "%s: not a state file", path);
+    if (h->version != 1) return nbh::fail(NBH_ERR_ARG, "%s: unsupported version %u", path, h->version);
+    if (h->n < 1 || h->n > (1LL << 31) - 1) return nbh::fail(NBH_ERR_ARG, "%s: bad body count %lld", path, (long long)h->n);
+    return NBH_OK;
+}
+
+}  // namespace
+
+int nbh_state_save(const char* path, int n, const double* x, const double* y, const double* z, const double* m,
+                   const double* vx, const double* vy, const double* vz, double time, int64_t step)
+{
+    const double* a[7] = {x, y, z, m, vx, vy, vz};
+    if (!path || n < 1) return nbh::fail(NBH_ERR_ARG, "nbh_state_save: need a path and n >= 1");
+    for (auto p : a)
+        if (!p) return nbh::fail(NBH_ERR_ARG, "nbh_state_save: array is NULL");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return nbh::fail(NBH_ERR_ARG, "%s: cannot open for writing", path);
+    StateHeader h{};
+    std::memcpy(h.magic, kMagic, 8);
+    h.version = 1;
+    h.n = n;
+    h.time = time;
+    h.step = step;
+    bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+    for (int k = 0; k < 7 && ok; ++k) ok = std::fwrite(a[k], sizeof(double), (size_t)n, f) == (size_t)n;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? NBH_OK : nbh::fail(NBH_ERR_ARG, "%s: write failed", path);
+}
+
+int nbh_state_info(const char* path, int* n, double* time, int64_t* step)
+{
+    if (!path) return nbh::fail(NBH_ERR_ARG, "nbh_state_info: path is NULL");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return nbh::fail(NBH_ERR_ARG, "%s: cannot open", path);
+    StateHeader h;
+    const int rc = read_header(f, path, &h);
+    std::fclose(f);
+    if (rc) return rc;
+    if (n) *n = (int)h.n;
+    if (time) *time = h.time;
+    if (step) *step = h.step;
+    return NBH_OK;
+}
+
+int nbh_state_load(const char* path, int n, double* x, double* y, double* z, double* m, double* vx, double* vy,
+                   double* vz)
+{
+    double* a[7] = {x, y, z, m, vx, vy, vz};
+    if (!path) return nbh::fail(NBH_ERR_ARG, "nbh_state_load: path is NULL");
+    for (auto p : a)
+        if (!p) return nbh::fail(NBH_ERR_ARG, "nbh_state_load: array is NULL");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return nbh::fail(NBH_ERR_ARG, "%s: cannot open", path);
+    StateHeader h;
+    int rc = read_header(f, path, &h);
+    if (rc == NBH_OK && h.n != n)
+        rc = nbh::fail(NBH_ERR_ARG, "%s holds %lld bodies, caller expects %d", path, (long long)h.n, n);
+    for (int k = 0; k < 7 && rc == NBH_OK; ++k)
+        if (std::fread(a[k], sizeof(double), (size_t)n, f) != (size_t)n)
+            rc = nbh::fail(NBH_ERR_ARG, "%s: truncated (array %d of 7)", path, k);
+    std::fclose(f);
+    return rc;
 }
 
 }  // extern "C"

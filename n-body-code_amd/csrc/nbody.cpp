@@ -71,7 +71,9 @@ NBody::~NBody()
 void NBody::initialize()
 {
     int rc;
-    if (ic == "plummer")
+    if (!load_path.empty())
+        rc = nbh_state_load(load_path.c_str(), N, h_par_x, h_par_y, h_par_z, h_par_w, h_vel_x, h_vel_y, h_vel_z);
+    else if (ic == "plummer")
         rc = nbh_plummer(N, seed, h_par_x, h_par_y, h_par_z, h_par_w, h_vel_x, h_vel_y, h_vel_z);
     else
         rc = nbh_reference_ic(N, vmax, rmax, mmin, mmax, h_par_x, h_par_y, h_par_z, h_par_w,
@@ -176,6 +178,7 @@ double NBody::updatedt()
 void NBody::timestepping()
 {
     if (fixed_dt > 0.0) dt = fixed_dt;
+    sim_time += dt;
     double t0 = now();
     prediction();
     sys.t_pred = now() - t0;
@@ -253,6 +256,9 @@ void NBody::cycle_resident(int ncycle)
     int64_t dummy;
     nbh_timing(c, &fms0, &dummy, &sms0, &dummy);
     for (int i = 0; i < ncycle; i++) {
+        double h = 0.0;
+        if (nbh_get_dt(c, &h) != NBH_OK) die("nbh_get_dt");
+        sim_time += h;
         if (nbh_step(c, 1, use_graph ? 1 : 0) != NBH_OK) die("nbh_step");
         double fms = 0.0, sms = 0.0;
         nbh_timing(c, &fms, &dummy, &sms, &dummy);

@@ -97,6 +97,8 @@ public:
     bool compat_energy = true;      // reference energy definition
     bool resident = false;          // whole step on the GPU
     bool use_graph = false;         // resident: replay steps from a hipGraph
+    std::string load_path;          // initial state from a state file (nbh_state_load)
+    double sim_time = 0.0;          // simulated time advanced by cycle()
 
     NBody(int Nparticles, double v_max, double r_max, double m_min, double m_max);
     ~NBody();

@@ -91,6 +91,9 @@ def lib():
     L.nbh_ens_info.argtypes = [vp, _I, _I, _I]
     L.nbh_reference_ic.argtypes = [C.c_int] + [C.c_double] * 4 + [_D] * 7
     L.nbh_force_vjp.argtypes = [vp, C.c_int, _D, _D, _D]
+    L.nbh_state_save.argtypes = [C.c_char_p, C.c_int] + [_D] * 7 + [C.c_double, C.c_int64]
+    L.nbh_state_info.argtypes = [C.c_char_p, C.POINTER(C.c_int), _D, C.POINTER(C.c_int64)]
+    L.nbh_state_load.argtypes = [C.c_char_p, C.c_int] + [_D] * 7
     L.nbh_adj_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
     L.nbh_adj_destroy.argtypes = [vp]
     L.nbh_adj_forward.argtypes = [vp, _D, C.c_double, C.c_int, _D]
@@ -130,6 +133,27 @@ def reference_ic(n, vmax=1.0, rmax=1.0, mmin=0.01, mmax=1.0):
     rc = lib().nbh_reference_ic(int(n), vmax, rmax, mmin, mmax, *[_p(s[k]) for k in range(7)])
     if rc != OK:
         raise NBodyError(rc, f"nbh_reference_ic(n={n}) rejected its arguments")
+    return s
+
+
+def state_save(path, state, time=0.0, step=0):
+    """Writes a (7, n) state to a body-state file (nbh_state_save)."""
+    s = _state(state)
+    _check(lib().nbh_state_save(os.fsencode(path), s.shape[1], *[_p(s[k]) for k in range(7)],
+                                float(time), int(step)))
+
+
+def state_info(path):
+    n, t, st = C.c_int(), C.c_double(), C.c_int64()
+    _check(lib().nbh_state_info(os.fsencode(path), C.byref(n), C.byref(t), C.byref(st)))
+    return {"n": n.value, "time": t.value, "step": st.value}
+
+
+def state_load(path):
+    """Reads a body-state file into a (7, n) array."""
+    n = state_info(path)["n"]
+    s = np.empty((7, n))
+    _check(lib().nbh_state_load(os.fsencode(path), n, *[_p(s[k]) for k in range(7)]))
     return s
 
 

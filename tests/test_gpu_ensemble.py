@@ -106,18 +106,24 @@ def test_steps_vs_oracle(nb, oracle, small):
 
 
 def test_graph_matches_eager(nb, small):
+    """Every replay of the captured 16-step ensemble graph is checked against
+    an eager twin, with eager energy evaluations between replays."""
     counts, eps2, state = small
-    res = []
-    for graph in (False, True):
-        with nb.Backend("mixed") as b, nb.Ensemble(b, counts, eps2) as e:
-            e.set_tuning([64, 128, 128, 256], [1, 2, 4, 8])
-            e.upload(state)
-            e.set_dt(1 / 256)
-            e.step(20, graph=graph)   # one 16-step graph + 4 eager steps
-            e.step(32, graph=graph)   # the graph replayed twice
-            res.append(e.download())
-    assert np.array_equal(res[0][0], res[1][0])
-    assert np.array_equal(res[0][1], res[1][1])
+    with nb.Backend("mixed") as bg, nb.Backend("mixed") as be, \
+            nb.Ensemble(bg, counts, eps2) as g, nb.Ensemble(be, counts, eps2) as e:
+        for x in (g, e):
+            x.set_tuning([64, 128, 128, 256], [1, 2, 4, 8])
+            x.upload(state)
+            x.set_dt(1 / 256)
+        for r in range(4):
+            g.step(16, graph=True)
+            e.step(16)
+            kg, ug = g.energy()
+            ke, ue = e.energy()
+            assert np.array_equal(kg, ke) and np.array_equal(ug, ue)
+            sg, fg = g.download()
+            se, fe = e.download()
+            assert np.array_equal(sg, se) and np.array_equal(fg, fe), f"replay {r}"
 
 
 def test_energy_per_system(nb, oracle, small):

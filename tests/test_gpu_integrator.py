@@ -103,18 +103,23 @@ def test_tiny_systems_vs_reference(nb, n):
 
 
 def test_graph_replay_matches_eager(nb):
+    """Every replay of the captured step graph is checked against an eager
+    twin, with eager work of the program (energy, downloads) in between."""
     s = nb.plummer(2000, 5)
-    out = []
-    for graph in (False, True, True):
-        with nb.Backend("fp64", 1e-4) as b:
+    with nb.Backend("fp64", 1e-4) as g, nb.Backend("fp64", 1e-4) as e:
+        for b in (g, e):
             b.upload(s)
             b.set_dt(1 / 1024)
-            b.step(40, graph=graph)   # one 32-step graph + 8 eager steps
-            b.step(64, graph=graph)   # the graph replayed twice
-            out.append(b.download())
-    for st, f in out[1:]:
-        assert np.array_equal(st, out[0][0])
-        assert np.array_equal(f, out[0][1])
+        for r in range(5):
+            g.step(32, graph=True)    # one replay of the 32-step graph
+            e.step(32, graph=False)
+            assert g.energy() == e.energy()
+            sg, fg = g.download()
+            se, fe = e.download()
+            assert np.array_equal(sg, se) and np.array_equal(fg, fe), f"replay {r}"
+        g.step(40, graph=True)        # one replay + 8 eager steps
+        e.step(40)
+        assert np.array_equal(g.download()[0], e.download()[0])
 
 
 def test_adaptive_graph_matches_eager(nb):
