@@ -41,6 +41,18 @@ enum nbh_precision {
                       two-sum compensated FP32 accumulation */
 };
 
+/* How pairs are evaluated.  DIRECTED: every ordered pair (i <- j) once, sums
+ * in a fixed order, bit-reproducible run to run.  SYMMETRIC: every unordered
+ * pair once, applied to both bodies (Newton's third law, as the reference's
+ * j < i loops at src/kernel.cpp:1126,1187), sums combined with FP64 atomic
+ * adds, reproducible to FP64 rounding only.  SYMMETRIC is implemented for
+ * NBH_MIXED and used when n is a multiple of 1024 and at least 2048; other n
+ * run DIRECTED (nbh_launch_pairing reports which). */
+enum nbh_pairing {
+    NBH_PAIR_DIRECTED = 0,
+    NBH_PAIR_SYMMETRIC = 1
+};
+
 typedef struct nbh_ctx nbh_ctx;
 
 typedef struct nbh_config {
@@ -52,6 +64,7 @@ typedef struct nbh_config {
                        64, 128 or 256; 0 = pick by size */
     int jsplit;     /* j-range split across workgroups; 0 = pick by size */
     int ipl;        /* targets per lane, 1 or 2; 0 = pick by size */
+    int pairing;    /* enum nbh_pairing; 0 = NBH_PAIR_DIRECTED */
 } nbh_config;
 
 /* Description of the last failure on this thread ("" if none). */
@@ -155,6 +168,8 @@ int nbh_launch_shape(nbh_ctx *ctx, int n, int *block, int *jsplit,
                      int *workgroups);
 /* Targets per lane of that shape. */
 int nbh_launch_ipl(nbh_ctx *ctx, int n, int *ipl);
+/* Pairing (enum nbh_pairing) that a force evaluation of n bodies uses. */
+int nbh_launch_pairing(nbh_ctx *ctx, int n, int *pairing);
 
 /* Autotuner of the single-system force: times block {64,128,256} x targets
  * per lane {1,2} x j-split {1..32} for n bodies on this context's device and

@@ -120,6 +120,15 @@ nbh::Shape make_shape(int n, int block, int ipl, int split, int nj = 0)
 nbh::Shape pick_shape(const nbh_ctx* c, int n)
 {
     const nbh_config& f = c->cfg;
+    if (f.pairing == NBH_PAIR_SYMMETRIC && f.precision == NBH_MIXED && nbh::sym_supported(n)) {
+        nbh::Shape s{};
+        s.block = 64 * 8;  // threads per workgroup of the symmetric kernel
+        s.ipl = 2;
+        s.jsplit = 1;
+        s.jchunk = n;
+        s.sym_chunk = nbh::sym_chunk(n, c->cus);
+        return s;
+    }
     if (f.block == 0 && f.jsplit == 0 && f.ipl == 0) {
         const auto& t = c->tuned[f.precision][size_bucket(n)];
         if (t.block) return make_shape(n, t.block, t.ipl, t.jsplit);
@@ -148,6 +157,10 @@ int check_cfg(const nbh_config* cfg)
         return fail(NBH_ERR_ARG, "block must be 0, 64, 128 or 256, got %d", cfg->block);
     if (cfg->jsplit < 0) return fail(NBH_ERR_ARG, "jsplit must be >= 0, got %d", cfg->jsplit);
     if (cfg->ipl < 0 || cfg->ipl > 2) return fail(NBH_ERR_ARG, "ipl must be 0, 1 or 2, got %d", cfg->ipl);
+    if (cfg->pairing != NBH_PAIR_DIRECTED && cfg->pairing != NBH_PAIR_SYMMETRIC)
+        return fail(NBH_ERR_ARG, "pairing must be NBH_PAIR_DIRECTED or NBH_PAIR_SYMMETRIC, got %d", cfg->pairing);
+    if (cfg->pairing == NBH_PAIR_SYMMETRIC && cfg->precision != NBH_MIXED)
+        return fail(NBH_ERR_ARG, "NBH_PAIR_SYMMETRIC is implemented for NBH_MIXED only");
     return NBH_OK;
 }
 
@@ -190,7 +203,7 @@ extern "C" {
 
 const char* nbh_last_error(void) { return nbh::last_error(); }
 
-int nbh_version(void) { return 100; }
+int nbh_version(void) { return 101; }
 
 int nbh_device_info(int device, int* compute_units, int* wave_size, char* name, int name_len)
 {
@@ -281,7 +294,22 @@ int nbh_launch_shape(nbh_ctx* c, int n, int* block, int* jsplit, int* workgroups
     const nbh::Shape s = pick_shape(c, n);
     if (block) *block = s.block;
     if (jsplit) *jsplit = s.jsplit;
-    if (workgroups) *workgroups = ((n + s.block * s.ipl - 1) / (s.block * s.ipl)) * s.jsplit;
+    if (workgroups) {
+        if (s.sym_chunk > 0) {
+            const int nb = n / nbh::sym_block();
+            *workgroups = nb * (((nb - 1) / 2 + s.sym_chunk) / s.sym_chunk) + n / 256;
+        } else {
+            *workgroups = ((n + s.block * s.ipl - 1) / (s.block * s.ipl)) * s.jsplit;
+        }
+    }
+    return NBH_OK;
+}
+
+int nbh_launch_pairing(nbh_ctx* c, int n, int* pairing)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    if (pairing) *pairing = pick_shape(c, n).sym_chunk > 0 ? NBH_PAIR_SYMMETRIC : NBH_PAIR_DIRECTED;
     return NBH_OK;
 }
 

@@ -38,7 +38,22 @@ struct PairArgs {
     double eps2;
     Forces out;
     size_t split_stride;  // element offset between j-split partial planes
+    int diag;             // > 0: sources are the target's own block of diag bodies
 };
+
+// Source range of this workgroup: its j-split, or in the block-diagonal pass
+// the block that holds its targets.
+template <int BLOCK, int IPL>
+__device__ __forceinline__ void source_range(const PairArgs& a, int& jbeg, int& jend)
+{
+    if (a.diag > 0) {
+        jbeg = (int)(blockIdx.x * BLOCK * IPL) / a.diag * a.diag;
+        jend = min(a.n, jbeg + a.diag);
+    } else {
+        jbeg = blockIdx.y * a.jchunk;
+        jend = min(a.nj, jbeg + a.jchunk);
+    }
+}
 
 // BLOCK lanes per workgroup, IPL targets per lane (lane t holds targets
 // base + t + q*BLOCK, q < IPL), so each broadcast source record read from
@@ -63,8 +78,8 @@ __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
         for (int c = 0; c < 6; ++c) acc[q][c] = 0.0;
     }
 
-    const int jbeg = blockIdx.y * a.jchunk;
-    const int jend = min(a.nj, jbeg + a.jchunk);
+    int jbeg, jend;
+    source_range<BLOCK, IPL>(a, jbeg, jend);
     JB64 st{};
     if (jbeg + t < jend) st = load_jb64(a.s, jbeg + t);
     tile[0][t] = st;
@@ -130,8 +145,8 @@ __global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a)
         for (int c = 0; c < 6; ++c) hi[q][c] = lo[q][c] = 0.f;
     }
 
-    const int jbeg = blockIdx.y * a.jchunk;
-    const int jend = min(a.nj, jbeg + a.jchunk);
+    int jbeg, jend;
+    source_range<BLOCK, IPL>(a, jbeg, jend);
     JBMX st{};
     if (jbeg + t < jend) st = load_jbmx(a.s, jbeg + t);
     tile[0][t] = st;
@@ -470,6 +485,11 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
     a.nj = nj > 0 ? nj : n;
     a.jchunk = shape.jchunk;
     a.eps2 = eps2;
+    a.diag = 0;
+    if (shape.sym_chunk > 0) {
+        if (precision != 1 || nj > 0) return hipErrorInvalidValue;
+        return launch_force_sym(s, n, eps2, shape.sym_chunk, out, st, ev0, ev1);
+    }
     if (shape.jsplit > 1) {
         // partial plane for (component c, split k) starts at (c*S + k)*n
         for (int c = 0; c < 6; ++c) a.out.c[c] = partial + (size_t)c * shape.jsplit * n;
@@ -498,6 +518,28 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
         e = hipGetLastError();
     }
     return e;
+}
+
+hipError_t launch_force_blockdiag(const State& s, int n, double eps2, int B,
+                                  const Forces& out, hipStream_t st)
+{
+    constexpr int BLOCK = 256;
+    if (B <= 0 || B % BLOCK != 0 || n % B != 0) return hipErrorInvalidValue;
+    PairArgs a;
+    a.s = s;
+    a.n = n;
+    a.nj = n;
+    a.jchunk = B;
+    a.eps2 = eps2;
+    a.out = out;
+    a.split_stride = 0;
+    a.diag = B;
+    const dim3 grid(n / BLOCK, 1);
+    if (eps2 == 0.0)
+        k_force_mixed<BLOCK, 1, true><<<grid, BLOCK, 0, st>>>(a);
+    else
+        k_force_mixed<BLOCK, 1, false><<<grid, BLOCK, 0, st>>>(a);
+    return hipGetLastError();
 }
 
 hipError_t launch_predict(int n, const double* dt, double* x, double* y, double* z,
