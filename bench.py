@@ -4,19 +4,24 @@
 Workload: N = 1,048,576 equal-mass Plummer sphere (seed 3) in Henon units,
 eps = 1e-3, dt = 1/2048, resident in HBM.  A step is one shared-timestep
 Hermite step (predict, pair force + jerk over all N^2 directed pairs,
-correct), run by the device-resident integrator.  Both precisions run:
-  FP64  -- the drop-in (plain FP64 pair arithmetic);
-  mixed -- FP64 position differences rounded to FP32, FP32 r^2 / rsqrt /
-           jerk, two-sum compensated FP32 sums (the headline `value`).
-`value` is the mixed path's N^2 * steps / time.  The mixed path is valid
-only where its median per-body relative acceleration error against FP64 is
-<= 1e-6; that error is measured in the same run on the same state.
+correct), run by the device-resident integrator.  Three legs run:
+  fp64           -- the drop-in (plain FP64 pair arithmetic, directed pairs);
+  mixed          -- FP64 position differences rounded to FP32, FP32 r^2 /
+                    rsqrt / jerk, two-sum compensated FP32 sums, each
+                    unordered pair evaluated once and applied to both bodies
+                    (symmetric pairing): the headline `value`;
+  mixed_directed -- the same arithmetic on directed pairs, deterministic.
+`value` is the mixed leg's N^2 * steps / time.  It is valid only where its
+median per-body relative acceleration error against FP64 is <= 1e-6; that
+error is measured in the same run on the same state.
 
-Roofline: the pair kernel is VALU-bound (no MFMA; ~6e5 flop per HBM byte).
-achieved = 60 flop per directed pair (acc + jerk convention) * N^2 / the
-kernel's mean launch time from HIP events recorded on its own stream in the
-timed region; peak = MI355X vector peak (FP32 157.3 TFLOP/s for the mixed
-kernel, FP64 78.6 TFLOP/s for the FP64 kernel).
+Roofline: the pair kernels are VALU-bound (no MFMA; ~1e4-1e5 flop per HBM
+byte).  Algorithmic work: 60 flop per directed pair (acc + jerk convention);
+the symmetric kernel does 73 flop per unordered pair (the 60 plus m_i r^3
+and six multiply-adds for the reaction), and its block-diagonal pass 60 per
+directed pair.  achieved = work / the kernels' mean launch time from HIP
+events recorded on their own stream in the timed region; peak = MI355X
+vector peak (FP32 157.3 TFLOP/s for mixed, FP64 78.6 TFLOP/s for FP64).
 
 cpu_baseline: the oracle's restatement of the reference FP64 force loop
 (src/kernel.cpp:1164-1224: Newton-3rd-law triangle, cyclic rows per thread,
@@ -39,6 +44,8 @@ SEED = 3
 EPS = 1e-3
 DT = 1.0 / 2048
 FLOP_PER_PAIR = 60
+FLOP_PER_SYM_PAIR = 73
+SYM_BLOCK = 512   # bodies per block of the symmetric kernel
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector
 
 
@@ -79,15 +86,29 @@ def measured_traffic():
 
 
 def kname(prec, res):
-    """Pair-kernel instance of a run: k_force_{f64,mixed}<block,targets per lane>."""
+    """Pair-kernel instance of a run: k_force_{f64,mixed}<block,targets per lane>
+    or the symmetric kernel k_force_mixed_sym<targets per lane, waves>."""
     sh = res["shape"]
+    if sh.get("pairing") == "symmetric":
+        return f"k_force_mixed_sym<{sh['ipl']},{sh['block'] // 64}>"
     return f"k_force_{'f64' if prec == 'fp64' else 'mixed'}<{sh['block']},{sh['ipl']}>"
 
 
-def run_precision(nb, prec, state, steps, warmup, log):
-    with nb.Backend(prec, EPS * EPS) as b:
-        tune = b.autotune(N)            # launch shape for this size bucket, timed on the device
-        log(f"{prec}: autotuned shape {tune}")
+def kernel_flop(shape):
+    """Algorithmic flop of one force evaluation for the launch shape used."""
+    if shape.get("pairing") == "symmetric":
+        diag = N * SYM_BLOCK - N                      # directed pairs inside blocks
+        return FLOP_PER_SYM_PAIR * (N * N - N * SYM_BLOCK) // 2 + FLOP_PER_PAIR * diag
+    return FLOP_PER_PAIR * N * N
+
+
+def run_precision(nb, prec, state, steps, warmup, log, pairing="directed"):
+    tag = prec if pairing == "directed" or prec == "fp64" else f"{prec}/{pairing}"
+    with nb.Backend(prec, EPS * EPS, pairing=pairing) as b:
+        tune = None
+        if b.launch_shape(N)["pairing"] == "directed":
+            tune = b.autotune(N)        # launch shape for this size bucket, timed on the device
+        log(f"{tag}: shape {b.launch_shape(N)}, autotune {tune}")
         b.upload(state)                 # initial force, NBody::cycle
         f0 = b.download()[1]
         b.set_dt(DT)
@@ -110,22 +131,32 @@ def run_precision(nb, prec, state, steps, warmup, log):
         dropin_wall = time.perf_counter() - t2
     sec = t1 - t0
     kms = tm["force_ms"] / max(tm["force_launches"], 1)
+    flop = kernel_flop(shape)
     res = {
         "pairs_per_s": N * N * steps / sec,
         "ms_per_step": 1e3 * sec / steps,
         "steps_per_s": steps / sec,
         "kernel_ms": kms,
         "kernel_launches": tm["force_launches"],
-        "kernel_tflops": FLOP_PER_PAIR * N * N / (kms * 1e-3) / 1e12,
+        "kernel_flop": flop,
+        "kernel_tflops": flop / (kms * 1e-3) / 1e12,
         "energy_drift": abs(e1 - e0) / abs(e0),
         "shape": shape,
         "autotune": tune,
         "dropin": {"wall_ms": 1e3 * dropin_wall, "kernel_ms": dropin_kernel_ms,
                    "pairs_per_s_pcie_inclusive": N * N / dropin_wall},
     }
-    log(f"{prec}: {res['ms_per_step']:.1f} ms/step, kernel {kms:.1f} ms, "
+    log(f"{tag}: {res['ms_per_step']:.1f} ms/step, kernel {kms:.1f} ms, "
         f"{res['pairs_per_s']:.3e} pairs/s, drift {res['energy_drift']:.2e}")
     return res, f0
+
+
+def roofline(prec, res, traffic):
+    return {"bound": "valu", "achieved": res["kernel_tflops"], "peak": PEAK[prec],
+            "unit": "TFLOP/s", "frac": res["kernel_tflops"] / PEAK[prec],
+            "traffic": traffic.get(kname(prec, res)),
+            "kernel": kname(prec, res), "kernel_ms": res["kernel_ms"],
+            "flop_per_launch": res["kernel_flop"]}
 
 
 def run_ensemble(nb, log, nsteps=64):
@@ -240,10 +271,13 @@ def main():
     log(f"Plummer N={N} in {time.perf_counter() - t:.1f} s")
 
     r64, f64 = run_precision(nb, "fp64", state, args.steps, args.warmup, log)
-    rmx, fmx = run_precision(nb, "mixed", state, args.steps, args.warmup, log)
+    rmx, fmx = run_precision(nb, "mixed", state, args.steps, args.warmup, log, pairing="symmetric")
+    rmd, fmd = run_precision(nb, "mixed", state, args.steps, args.warmup, log)
     err = body_relerr(fmx[:3], f64[:3])
     med, mx = float(np.median(err)), float(err.max())
-    log(f"mixed vs fp64 acc rel err: median {med:.2e}, max {mx:.2e}")
+    errd = body_relerr(fmd[:3], f64[:3])
+    log(f"mixed vs fp64 acc rel err: median {med:.2e}, max {mx:.2e} "
+        f"(directed: median {np.median(errd):.2e}, max {errd.max():.2e})")
 
     ens = None if args.no_ensemble else run_ensemble(nb, log)
     adj = None if args.no_ensemble else run_adjoint(nb, log)
@@ -274,20 +308,19 @@ def main():
         "mixed_max_rel_acc_err": mx,
         "fp64_pairs_per_s": r64["pairs_per_s"],
         "mixed_over_fp64": rmx["pairs_per_s"] / r64["pairs_per_s"],
-        "steps_per_s": {"fp64": r64["steps_per_s"], "mixed": rmx["steps_per_s"]},
-        "energy_drift": {"fp64": r64["energy_drift"], "mixed": rmx["energy_drift"]},
+        "mixed_pairing": rmx["shape"]["pairing"],
+        "mixed_directed": {"pairs_per_s": rmd["pairs_per_s"], "over_fp64": rmd["pairs_per_s"] / r64["pairs_per_s"],
+                           "median_rel_acc_err": float(np.median(errd)), "max_rel_acc_err": float(errd.max()),
+                           "energy_drift": rmd["energy_drift"], "launch_shape": rmd["shape"],
+                           "roofline": roofline("mixed", rmd, traffic)},
+        "steps_per_s": {"fp64": r64["steps_per_s"], "mixed": rmx["steps_per_s"],
+                        "mixed_directed": rmd["steps_per_s"]},
+        "energy_drift": {"fp64": r64["energy_drift"], "mixed": rmx["energy_drift"],
+                         "mixed_directed": rmd["energy_drift"]},
         "launch_shape": rmx["shape"],
         "dropin_host_arrays": {"fp64": r64["dropin"], "mixed": rmx["dropin"]},
-        "roofline": {"bound": "valu", "achieved": rmx["kernel_tflops"], "peak": PEAK["mixed"],
-                     "unit": "TFLOP/s", "frac": rmx["kernel_tflops"] / PEAK["mixed"],
-                     "traffic": traffic.get(kname("mixed", rmx)),
-                     "kernel": kname("mixed", rmx), "kernel_ms": rmx["kernel_ms"],
-                     "flop_per_launch": FLOP_PER_PAIR * N * N},
-        "roofline_fp64": {"bound": "valu", "achieved": r64["kernel_tflops"], "peak": PEAK["fp64"],
-                          "unit": "TFLOP/s", "frac": r64["kernel_tflops"] / PEAK["fp64"],
-                          "traffic": traffic.get(kname("fp64", r64)),
-                          "kernel": kname("fp64", r64), "kernel_ms": r64["kernel_ms"],
-                          "flop_per_launch": FLOP_PER_PAIR * N * N},
+        "roofline": roofline("mixed", rmx, traffic),
+        "roofline_fp64": roofline("fp64", r64, traffic),
         "cpu_baseline": cpu,
         "ensemble": ens,
         "adjoint": adj,

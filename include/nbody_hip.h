@@ -46,7 +46,7 @@ enum nbh_precision {
  * pair once, applied to both bodies (Newton's third law, as the reference's
  * j < i loops at src/kernel.cpp:1126,1187), sums combined with FP64 atomic
  * adds, reproducible to FP64 rounding only.  SYMMETRIC is implemented for
- * NBH_MIXED and used when n is a multiple of 1024 and at least 2048; other n
+ * NBH_MIXED and used when n is a multiple of 512 and at least 1024; other n
  * run DIRECTED (nbh_launch_pairing reports which). */
 enum nbh_pairing {
     NBH_PAIR_DIRECTED = 0,
@@ -174,10 +174,12 @@ int nbh_launch_pairing(nbh_ctx *ctx, int n, int *pairing);
 /* Autotuner of the single-system force: times block {64,128,256} x targets
  * per lane {1,2} x j-split {1..32} for n bodies on this context's device and
  * precision (a Plummer sphere, on a source range of max(min(n, 65536), n/16)
- * bodies so that a trial is short) and keeps the fastest for n's size
- * bucket (floor(log2 n)) and precision.  Later evaluations in that bucket use
- * it unless the config forces block/jsplit/ipl.  Outputs may be NULL; ms is
- * the winner's trial time. */
+ * bodies so that a trial is short; when that range is truncated, the three
+ * fastest shapes are timed again on all n sources) and keeps the fastest
+ * for n's size bucket (floor(log2 n)) and precision.  Later evaluations in
+ * that bucket use it unless the config forces block/jsplit/ipl.  Directed
+ * pairing only: the symmetric kernel has a fixed shape.  Outputs may be
+ * NULL; ms is the winner's last timed launch. */
 int nbh_autotune(nbh_ctx *ctx, int n, int *block, int *ipl, int *jsplit,
                  double *ms);
 

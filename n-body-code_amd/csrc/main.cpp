@@ -18,7 +18,8 @@ constexpr int kCol = 15;
 const char* kUsage =
     "Usage: ./nbody_app [--N <value>] [--Ncycle <value>]\n"
     "  extensions: [--eps <softening>] [--dt <fixed step>] [--ic ref|plummer] [--seed <int>]\n"
-    "              [--precision fp64|mixed] [--resident] [--graph] [--energy compat|physical]\n"
+    "              [--precision fp64|mixed] [--pairing directed|symmetric] [--resident] [--graph]\n"
+    "              [--energy compat|physical]\n"
     "              [--device <ordinal>] [--load <state file>] [--save <state file>]";
 
 int positive_int(const std::string& name, const char* text)
@@ -70,6 +71,11 @@ int main(int argc, char* argv[])
                 if (p == "fp64") fb.precision = NBH_FP64;
                 else if (p == "mixed") fb.precision = NBH_MIXED;
                 else throw std::invalid_argument("precision must be fp64 or mixed.");
+            } else if (arg == "--pairing" && has) {
+                const std::string p = argv[++i];
+                if (p == "directed") fb.pairing = NBH_PAIR_DIRECTED;
+                else if (p == "symmetric") fb.pairing = NBH_PAIR_SYMMETRIC;
+                else throw std::invalid_argument("pairing must be directed or symmetric.");
             } else if (arg == "--energy" && has) {
                 energy = argv[++i];
                 if (energy != "compat" && energy != "physical")
@@ -95,6 +101,10 @@ int main(int argc, char* argv[])
         }
     }
 
+    if (fb.pairing == NBH_PAIR_SYMMETRIC && fb.precision != NBH_MIXED) {
+        std::cerr << "Error: --pairing symmetric requires --precision mixed" << std::endl;
+        return 1;
+    }
     if (!load_path.empty()) {
         int nf = 0;
         if (nbh_state_info(load_path.c_str(), &nf, nullptr, nullptr) != NBH_OK) {

@@ -122,7 +122,7 @@ nbh::Shape pick_shape(const nbh_ctx* c, int n)
     const nbh_config& f = c->cfg;
     if (f.pairing == NBH_PAIR_SYMMETRIC && f.precision == NBH_MIXED && nbh::sym_supported(n)) {
         nbh::Shape s{};
-        s.block = 64 * 8;  // threads per workgroup of the symmetric kernel
+        s.block = nbh::sym_block() / 2;  // threads per workgroup (2 targets per lane)
         s.ipl = 2;
         s.jsplit = 1;
         s.jchunk = n;
@@ -341,9 +341,31 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
     hipEvent_t e0, e1;
     NBH_HIP(hipEventCreate(&e0));
     NBH_HIP(hipEventCreate(&e1));
-    double best = 1e300;
-    int bb = 0, bi = 0, bs = 0;
+    struct Trial {
+        double ms;
+        int block, ipl, split;
+    };
+    std::vector<Trial> trials;
     int rc = NBH_OK;
+    // one timed launch (after `reps - 1` untimed ones) of shape sh
+    auto time_shape = [&](const nbh::Shape& sh, int reps, int trial_nj, double* out_ms) -> int {
+        if (sh.jsplit > 1 && c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n) != hipSuccess)
+            return fail(NBH_ERR_ALLOC, "autotune: partial-sum allocation failed");
+        for (int rep = 0; rep < reps; ++rep) {
+            hipError_t e = hipEventRecord(e0, c->stream);
+            if (e == hipSuccess)
+                e = nbh::launch_force(c->cfg.precision, st, n, c->cfg.eps2, sh, fo, c->partial.d(), c->stream,
+                                      nullptr, nullptr, trial_nj);
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) return fail(NBH_ERR_HIP, "autotune trial failed: %s", hipGetErrorString(e));
+            *out_ms = ms;
+        }
+        return (int)NBH_OK;
+    };
+    // stage 1: every shape on the truncated source range
     for (int block : {64, 128, 256}) {
         if (n >= 65536 && block == 64) continue;
         for (int ipl : {1, 2}) {
@@ -352,34 +374,32 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
                 const nbh::Shape sh = make_shape(n, block, ipl, split, nj);
                 if (sh.jsplit == last) continue;
                 last = sh.jsplit;
-                if (sh.jsplit > 1 && c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n) != hipSuccess) {
-                    rc = fail(NBH_ERR_ALLOC, "autotune: partial-sum allocation failed");
-                    break;
-                }
-                double mn = 1e300;
-                for (int rep = 0; rep < 2 && rc == NBH_OK; ++rep) {
-                    hipError_t e = hipEventRecord(e0, c->stream);
-                    if (e == hipSuccess)
-                        e = nbh::launch_force(c->cfg.precision, st, n, c->cfg.eps2, sh, fo, c->partial.d(), c->stream,
-                                              nullptr, nullptr, nj);
-                    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-                    if (e == hipSuccess) e = hipEventSynchronize(e1);
-                    float ms = 0.f;
-                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                    if (e != hipSuccess) rc = fail(NBH_ERR_HIP, "autotune trial failed: %s", hipGetErrorString(e));
-                    if (rep > 0) mn = ms;  // the first launch warms up
-                }
-                if (rc) break;
-                if (mn < best) {
-                    best = mn;
-                    bb = block;
-                    bi = ipl;
-                    bs = split;
-                }
+                double ms = 0.0;
+                if ((rc = time_shape(sh, 2, nj, &ms)) != NBH_OK) break;  // the first launch warms up
+                trials.push_back({ms, block, ipl, split});
             }
             if (rc) break;
         }
         if (rc) break;
+    }
+    double best = 1e300;
+    int bb = 0, bi = 0, bs = 0;
+    if (rc == NBH_OK) {
+        std::sort(trials.begin(), trials.end(), [](const Trial& a, const Trial& b) { return a.ms < b.ms; });
+        // stage 2 (truncated trials only): the three fastest again on all n
+        // sources, since the truncation changes how the splits balance
+        const bool full = nj < n;
+        const size_t top = full ? std::min<size_t>(3, trials.size()) : std::min<size_t>(1, trials.size());
+        for (size_t k = 0; k < top && rc == NBH_OK; ++k) {
+            double ms = trials[k].ms;
+            if (full) rc = time_shape(make_shape(n, trials[k].block, trials[k].ipl, trials[k].split), 1, 0, &ms);
+            if (rc == NBH_OK && ms < best) {
+                best = ms;
+                bb = trials[k].block;
+                bi = trials[k].ipl;
+                bs = trials[k].split;
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

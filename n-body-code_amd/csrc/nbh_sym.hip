@@ -43,7 +43,7 @@ namespace nbh {
 namespace {
 
 constexpr int kSymK = 2;     // targets per lane
-constexpr int kSymW = 8;     // waves per workgroup
+constexpr int kSymW = 4;     // waves per workgroup
 constexpr int kSymTile = 64; // sources per LDS tile (4 groups of 16)
 
 // Row rotation by R lanes: lane k of each 16-lane row receives the value of

@@ -45,6 +45,7 @@ struct ForceBackend {
     int device = 0;
     int precision = NBH_FP64;
     double eps2 = 1.0e-7;  // the reference's EPS2 (include/kernel.h:13)
+    int pairing = NBH_PAIR_DIRECTED;
 };
 
 // Sets (or changes) the backend; the context is opened on first use.

@@ -20,8 +20,8 @@ def force(nb, state, eps2, pairing):
         return b.force(state), shape
 
 
-# n = 2048, 3072, 4096, 5120: 2, 3, 4, 5 blocks of 1024 (even and odd ring)
-@pytest.mark.parametrize("n", [2048, 3072, 4096, 5120])
+# n = 1024 .. 2560: 2, 3, 4, 5 blocks of 512 (even and odd ring)
+@pytest.mark.parametrize("n", [1024, 1536, 2048, 2560])
 def test_symmetric_matches_oracle(nb, oracle, n):
     _banner(f"oracle n={n}")
     s = nb.plummer(n, 11 + n)
@@ -76,7 +76,7 @@ def test_eps2_zero_diagonal_pass(nb, oracle):
 
 
 def test_unsupported_n_runs_directed(nb):
-    for n in (1000, 1024, 2047, 3000):
+    for n in (1000, 512, 1023, 3000):
         s = nb.plummer(n, 2)
         fs, shape = force(nb, s, 1e-4, "symmetric")
         assert shape["pairing"] == "directed"
