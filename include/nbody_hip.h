@@ -100,7 +100,9 @@ int nbh_force(nbh_ctx *ctx, int n,
 /* Same evaluation on device-resident arrays (HBM pointers), enqueued on
  * `stream` (a hipStream_t; NULL = the context's own stream), asynchronous.
  * No counterpart in the reference (its arrays are host memory); it lets a
- * caller that already holds device buffers skip the copies. */
+ * caller that already holds device buffers skip the copies.  With
+ * NBH_PAIR_SYMMETRIC the outputs receive FP64 atomic adds, so they must be
+ * ordinary device allocations (hipMalloc), not host or managed memory. */
 int nbh_force_device(nbh_ctx *ctx, int n,
                      const double *x, const double *y, const double *z,
                      const double *m, const double *vx, const double *vy,
