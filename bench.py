@@ -4,16 +4,21 @@
 Workload: N = 1,048,576 equal-mass Plummer sphere (seed 3) in Henon units,
 eps = 1e-3, dt = 1/2048, resident in HBM.  A step is one shared-timestep
 Hermite step (predict, pair force + jerk over all N^2 directed pairs,
-correct), run by the device-resident integrator.  Three legs run:
-  fp64           -- the drop-in (plain FP64 pair arithmetic, directed pairs);
+correct), run by the device-resident integrator.  Four legs run:
+  fp64           -- the drop-in (plain FP64 pair arithmetic, directed pairs,
+                    bit-reproducible);
+  fp64_symmetric -- FP64, each unordered pair evaluated once and applied to
+                    both bodies (symmetric pairing, the reference's j < i
+                    scheme), totals combined with FP64 atomics;
   mixed          -- FP64 position differences rounded to FP32, FP32 r^2 /
-                    rsqrt / jerk, two-sum compensated FP32 sums, each
-                    unordered pair evaluated once and applied to both bodies
-                    (symmetric pairing): the headline `value`;
-  mixed_directed -- the same arithmetic on directed pairs, deterministic.
+                    rsqrt / jerk, two-sum compensated FP32 sums, symmetric
+                    pairing: the headline `value`;
+  mixed_directed -- the mixed arithmetic on directed pairs.
 `value` is the mixed leg's N^2 * steps / time.  It is valid only where its
 median per-body relative acceleration error against FP64 is <= 1e-6; that
-error is measured in the same run on the same state.
+error is measured in the same run on the same state.  mixed_over_fp64 gives
+the like-for-like ratios (symmetric/symmetric, directed/directed) and the
+symmetric mixed leg over the directed FP64 drop-in.
 
 Roofline: the pair kernels are VALU-bound (no MFMA; ~1e4-1e5 flop per HBM
 byte).  Algorithmic work: 60 flop per directed pair (acc + jerk convention);
@@ -90,7 +95,7 @@ def kname(prec, res):
     or the symmetric kernel k_force_mixed_sym<targets per lane, waves>."""
     sh = res["shape"]
     if sh.get("pairing") == "symmetric":
-        return f"k_force_mixed_sym<{sh['ipl']},{sh['block'] // 64}>"
+        return f"k_force_sym<{'SymF64' if prec == 'fp64' else 'SymMixed'},{sh['ipl']},{sh['block'] // 64}>"
     return f"k_force_{'f64' if prec == 'fp64' else 'mixed'}<{sh['block']},{sh['ipl']}>"
 
 
@@ -271,13 +276,17 @@ def main():
     log(f"Plummer N={N} in {time.perf_counter() - t:.1f} s")
 
     r64, f64 = run_precision(nb, "fp64", state, args.steps, args.warmup, log)
+    r64s, f64s = run_precision(nb, "fp64", state, args.steps, args.warmup, log, pairing="symmetric")
     rmx, fmx = run_precision(nb, "mixed", state, args.steps, args.warmup, log, pairing="symmetric")
     rmd, fmd = run_precision(nb, "mixed", state, args.steps, args.warmup, log)
     err = body_relerr(fmx[:3], f64[:3])
     med, mx = float(np.median(err)), float(err.max())
     errd = body_relerr(fmd[:3], f64[:3])
+    # symmetric vs directed FP64, per body, acceleration and jerk
+    err64s = np.maximum(body_relerr(f64s[:3], f64[:3]), body_relerr(f64s[3:], f64[3:]))
     log(f"mixed vs fp64 acc rel err: median {med:.2e}, max {mx:.2e} "
-        f"(directed: median {np.median(errd):.2e}, max {errd.max():.2e})")
+        f"(directed: median {np.median(errd):.2e}, max {errd.max():.2e}); "
+        f"fp64 symmetric vs directed max {err64s.max():.2e}")
 
     ens = None if args.no_ensemble else run_ensemble(nb, log)
     adj = None if args.no_ensemble else run_adjoint(nb, log)
@@ -307,16 +316,21 @@ def main():
         "mixed_median_rel_acc_err": med,
         "mixed_max_rel_acc_err": mx,
         "fp64_pairs_per_s": r64["pairs_per_s"],
-        "mixed_over_fp64": rmx["pairs_per_s"] / r64["pairs_per_s"],
+        "mixed_over_fp64": {"symmetric_over_symmetric": rmx["pairs_per_s"] / r64s["pairs_per_s"],
+                            "directed_over_directed": rmd["pairs_per_s"] / r64["pairs_per_s"],
+                            "symmetric_over_directed_dropin": rmx["pairs_per_s"] / r64["pairs_per_s"]},
         "mixed_pairing": rmx["shape"]["pairing"],
+        "fp64_symmetric": {"pairs_per_s": r64s["pairs_per_s"], "over_fp64_directed": r64s["pairs_per_s"] / r64["pairs_per_s"],
+                           "max_rel_err_vs_directed": float(err64s.max()), "energy_drift": r64s["energy_drift"],
+                           "launch_shape": r64s["shape"], "roofline": roofline("fp64", r64s, traffic)},
         "mixed_directed": {"pairs_per_s": rmd["pairs_per_s"], "over_fp64": rmd["pairs_per_s"] / r64["pairs_per_s"],
                            "median_rel_acc_err": float(np.median(errd)), "max_rel_acc_err": float(errd.max()),
                            "energy_drift": rmd["energy_drift"], "launch_shape": rmd["shape"],
                            "roofline": roofline("mixed", rmd, traffic)},
-        "steps_per_s": {"fp64": r64["steps_per_s"], "mixed": rmx["steps_per_s"],
-                        "mixed_directed": rmd["steps_per_s"]},
-        "energy_drift": {"fp64": r64["energy_drift"], "mixed": rmx["energy_drift"],
-                         "mixed_directed": rmd["energy_drift"]},
+        "steps_per_s": {"fp64": r64["steps_per_s"], "fp64_symmetric": r64s["steps_per_s"],
+                        "mixed": rmx["steps_per_s"], "mixed_directed": rmd["steps_per_s"]},
+        "energy_drift": {"fp64": r64["energy_drift"], "fp64_symmetric": r64s["energy_drift"],
+                         "mixed": rmx["energy_drift"], "mixed_directed": rmd["energy_drift"]},
         "launch_shape": rmx["shape"],
         "dropin_host_arrays": {"fp64": r64["dropin"], "mixed": rmx["dropin"]},
         "roofline": roofline("mixed", rmx, traffic),

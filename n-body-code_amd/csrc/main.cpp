@@ -101,10 +101,6 @@ int main(int argc, char* argv[])
         }
     }
 
-    if (fb.pairing == NBH_PAIR_SYMMETRIC && fb.precision != NBH_MIXED) {
-        std::cerr << "Error: --pairing symmetric requires --precision mixed" << std::endl;
-        return 1;
-    }
     if (!load_path.empty()) {
         int nf = 0;
         if (nbh_state_info(load_path.c_str(), &nf, nullptr, nullptr) != NBH_OK) {

@@ -120,13 +120,13 @@ nbh::Shape make_shape(int n, int block, int ipl, int split, int nj = 0)
 nbh::Shape pick_shape(const nbh_ctx* c, int n)
 {
     const nbh_config& f = c->cfg;
-    if (f.pairing == NBH_PAIR_SYMMETRIC && f.precision == NBH_MIXED && nbh::sym_supported(n)) {
+    if (f.pairing == NBH_PAIR_SYMMETRIC && nbh::sym_supported(f.precision, n)) {
         nbh::Shape s{};
-        s.block = nbh::sym_block() / 2;  // threads per workgroup (2 targets per lane)
+        s.block = nbh::sym_block(f.precision) / 2;  // threads per workgroup (2 targets per lane)
         s.ipl = 2;
         s.jsplit = 1;
         s.jchunk = n;
-        s.sym_chunk = nbh::sym_chunk(n, c->cus);
+        s.sym_chunk = nbh::sym_chunk(f.precision, n, c->cus);
         return s;
     }
     if (f.block == 0 && f.jsplit == 0 && f.ipl == 0) {
@@ -159,8 +159,6 @@ int check_cfg(const nbh_config* cfg)
     if (cfg->ipl < 0 || cfg->ipl > 2) return fail(NBH_ERR_ARG, "ipl must be 0, 1 or 2, got %d", cfg->ipl);
     if (cfg->pairing != NBH_PAIR_DIRECTED && cfg->pairing != NBH_PAIR_SYMMETRIC)
         return fail(NBH_ERR_ARG, "pairing must be NBH_PAIR_DIRECTED or NBH_PAIR_SYMMETRIC, got %d", cfg->pairing);
-    if (cfg->pairing == NBH_PAIR_SYMMETRIC && cfg->precision != NBH_MIXED)
-        return fail(NBH_ERR_ARG, "NBH_PAIR_SYMMETRIC is implemented for NBH_MIXED only");
     return NBH_OK;
 }
 
@@ -296,7 +294,7 @@ int nbh_launch_shape(nbh_ctx* c, int n, int* block, int* jsplit, int* workgroups
     if (jsplit) *jsplit = s.jsplit;
     if (workgroups) {
         if (s.sym_chunk > 0) {
-            const int nb = n / nbh::sym_block();
+            const int nb = n / nbh::sym_block(c->cfg.precision);
             *workgroups = nb * (((nb - 1) / 2 + s.sym_chunk) / s.sym_chunk) + n / 256;
         } else {
             *workgroups = ((n + s.block * s.ipl - 1) / (s.block * s.ipl)) * s.jsplit;

@@ -31,7 +31,7 @@ struct Shape {
     int ipl;     // targets per lane (1 or 2): block*ipl targets per workgroup
     int jsplit;  // workgroups along j; partial sums reduced in a second pass
     int jchunk;  // j-bodies per split (multiple of block)
-    int sym_chunk;  // > 0: symmetric mixed kernel, block pairs per workgroup
+    int sym_chunk;  // > 0: symmetric kernel, block pairs per workgroup
 };
 
 // Pair-kernel launch: writes forces for all n targets from all n sources
@@ -47,18 +47,19 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
 // Directed pair kernel restricted to the diagonal blocks: target i gets the
 // sources of its own block [B*(i/B), B*(i/B)+B), self-pair excluded; the
 // outputs are overwritten.  n % B == 0 and B % 256 == 0.
-hipError_t launch_force_blockdiag(const State& s, int n, double eps2, int B,
+hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
                                   const Forces& out, hipStream_t st);
 
-// Symmetric mixed-precision pair kernel (nbh_sym.hip): block-diagonal pass
-// then every off-diagonal block pair once, reactions added to both sides.
-// Usable when sym_supported(n); ev0/ev1 (may be null) bracket both launches.
-int sym_block();
-bool sym_supported(int n);
-int sym_chunk(int n, int cus);
-hipError_t launch_force_sym(const State& s, int n, double eps2, int chunk,
-                            const Forces& out, hipStream_t st, hipEvent_t ev0,
-                            hipEvent_t ev1);
+// Symmetric pair kernels (nbh_sym.hip): block-diagonal pass, then every
+// off-diagonal block pair once with the reaction added to the source side.
+// Usable when sym_supported(precision, n); ev0/ev1 (may be null) bracket
+// both launches.
+int sym_block(int precision);
+bool sym_supported(int precision, int n);
+int sym_chunk(int precision, int n, int cus);
+hipError_t launch_force_sym(int precision, const State& s, int n, double eps2,
+                            int chunk, const Forces& out, hipStream_t st,
+                            hipEvent_t ev0, hipEvent_t ev1);
 
 // Hermite predictor in place (x, v) from forces f0 with dt read from *dt.
 hipError_t launch_predict(int n, const double* dt, double* x, double* y,

@@ -487,8 +487,8 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
     a.eps2 = eps2;
     a.diag = 0;
     if (shape.sym_chunk > 0) {
-        if (precision != 1 || nj > 0) return hipErrorInvalidValue;
-        return launch_force_sym(s, n, eps2, shape.sym_chunk, out, st, ev0, ev1);
+        if (nj > 0) return hipErrorInvalidValue;
+        return launch_force_sym(precision, s, n, eps2, shape.sym_chunk, out, st, ev0, ev1);
     }
     if (shape.jsplit > 1) {
         // partial plane for (component c, split k) starts at (c*S + k)*n
@@ -520,7 +520,7 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
     return e;
 }
 
-hipError_t launch_force_blockdiag(const State& s, int n, double eps2, int B,
+hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
                                   const Forces& out, hipStream_t st)
 {
     constexpr int BLOCK = 256;
@@ -534,12 +534,7 @@ hipError_t launch_force_blockdiag(const State& s, int n, double eps2, int B,
     a.out = out;
     a.split_stride = 0;
     a.diag = B;
-    const dim3 grid(n / BLOCK, 1);
-    if (eps2 == 0.0)
-        k_force_mixed<BLOCK, 1, true><<<grid, BLOCK, 0, st>>>(a);
-    else
-        k_force_mixed<BLOCK, 1, false><<<grid, BLOCK, 0, st>>>(a);
-    return hipGetLastError();
+    return launch_pair<BLOCK, 1>(precision, eps2 == 0.0, a, 1, st);
 }
 
 hipError_t launch_predict(int n, const double* dt, double* x, double* y, double* z,

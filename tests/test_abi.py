@@ -93,10 +93,9 @@ def test_bad_config_rejected_before_device_use(nb):
     h = C.c_void_p()
     for cfg in (nb.Config(0, 7, 1e-7, 0, 0), nb.Config(0, 0, -1.0, 0, 0),
                 nb.Config(0, 0, 1e-7, 96, 0), nb.Config(0, 0, 1e-7, 0, -2),
-                nb.Config(0, 1, 1e-7, 0, 0, 0, 2),     # unknown pairing
-                nb.Config(0, 0, 1e-7, 0, 0, 0, 1)):    # symmetric pairing is mixed-only
+                nb.Config(0, 1, 1e-7, 0, 0, 0, 2)):    # unknown pairing
         assert L.nbh_create(C.byref(cfg), C.byref(h)) == nb.ERR_ARG
-    assert b"NBH_MIXED only" in L.nbh_last_error()
+    assert b"pairing must be" in L.nbh_last_error()
     assert L.nbh_create(None, C.byref(h)) == nb.ERR_ARG
 
 

@@ -26,7 +26,6 @@ def run(*args, timeout=120):
     (["--precision", "fp16"], "precision must be"),
     (["--dt", "0"], "dt must be positive"),
     (["--pairing", "both"], "pairing must be directed or symmetric"),
-    (["--pairing", "symmetric"], "requires --precision mixed"),
 ])
 def test_bad_arguments_exit_nonzero(args, needle):
     r = run(*args)
@@ -56,16 +55,17 @@ def test_reference_run_initial_energy_matches_reference():
 
 
 @pytest.mark.gpu
-def test_symmetric_pairing_run():
-    """--pairing symmetric: same run as directed mixed to the mixed tolerance."""
+@pytest.mark.parametrize("precision,tol", [("mixed", 1e-7), ("fp64", 1e-12)])
+def test_symmetric_pairing_run(precision, tol):
+    """--pairing symmetric: the same run as directed pairing, to the precision's tolerance."""
     common = ["--N", "2048", "--Ncycle", "4", "--ic", "plummer", "--seed", "6", "--eps", "0.01",
-              "--dt", "0.001", "--energy", "physical", "--precision", "mixed"]
+              "--dt", "0.001", "--energy", "physical", "--precision", precision]
     a = run(*common)
     b = run(*common, "--pairing", "symmetric", "--resident", "--graph")
     assert a.returncode == 0 and b.returncode == 0, a.stderr + b.stderr
     ea, eb = energies(a.stdout), energies(b.stdout)
     assert abs(ea[0] - eb[0]) <= 1e-12 * abs(ea[0])
-    assert abs(ea[1] - eb[1]) <= 1e-7 * abs(ea[1])
+    assert abs(ea[1] - eb[1]) <= tol * abs(ea[1])
 
 
 @pytest.mark.gpu
