@@ -45,9 +45,9 @@ enum nbh_precision {
  * in a fixed order, bit-reproducible run to run.  SYMMETRIC: every unordered
  * pair once, applied to both bodies (Newton's third law, as the reference's
  * j < i loops at src/kernel.cpp:1126,1187), sums combined with FP64 atomic
- * adds, reproducible to FP64 rounding only.  SYMMETRIC is implemented for
- * NBH_MIXED and used when n is a multiple of 512 and at least 1024; other n
- * run DIRECTED (nbh_launch_pairing reports which). */
+ * adds, reproducible to FP64 rounding only.  SYMMETRIC is used (in both
+ * precisions) when n is a multiple of 512 and at least 1024; other n run
+ * DIRECTED (nbh_launch_pairing reports which). */
 enum nbh_pairing {
     NBH_PAIR_DIRECTED = 0,
     NBH_PAIR_SYMMETRIC = 1
