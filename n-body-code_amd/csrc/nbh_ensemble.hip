@@ -6,8 +6,8 @@
 //
 // Force evaluation: one persistent launch of one-wave (64-lane) workgroups.
 // Each wave takes work items from a device queue with an atomic counter:
-// item = (system, 64 consecutive targets, a source range, an LDS tile
-// length, an output slot).  Items are sorted by cost, largest first, so the
+// item = (system, 64 or 128 consecutive targets (one or two per lane), a
+// source range, an LDS tile length, an output slot).  Items are sorted by cost, largest first, so the
 // long items start early and the short ones fill the tail.  Systems whose
 // bucket uses a j-split write per-item partial sums into slots that a second
 // kernel adds in split order (deterministic).  Every wave leaves the loop
@@ -28,6 +28,7 @@ namespace nbh {
 namespace {
 
 constexpr int kLanes = 64;
+constexpr int kMaxTargets = 128;  // targets per item: 64 per lane-set, up to two sets
 const int kBucketEdge[NBH_ENS_BUCKETS] = {64, 256, 1024, INT_MAX};
 const int kTiles[3] = {64, 128, 256};
 const int kSplits[4] = {1, 2, 4, 8};
@@ -41,11 +42,11 @@ struct EnsSys {
 struct EnsItem {
     int sys;
     int i0;      // first target (global index)
-    int icount;  // targets in this item (<= 64)
+    int icount;  // targets in this item (<= 64 * ipl)
     int j0, j1;  // source range (global indices)
     int tile;    // LDS tile length for this item
     int slot;    // partial-sum slot, or -1: write the forces directly
-    int pad;
+    int ipl;     // targets per lane: lane t holds i0 + t and, if 2, i0 + 64 + t
 };
 
 struct EnsReduce {
@@ -62,7 +63,7 @@ struct EnsArgs {
     const EnsItem* items;
     int nitems;
     unsigned* counter;
-    double* partial;  // [slot][6][64]
+    double* partial;  // [slot][6][kMaxTargets]
 };
 
 int bucket_of(int n)
@@ -81,77 +82,142 @@ __device__ __forceinline__ int take_item(unsigned* counter)
     return __builtin_amdgcn_readfirstlane(w);
 }
 
+// One work item with IPL targets per lane: sources streamed through the LDS
+// tile, broadcast to the lanes, each record feeding IPL pair evaluations.
+template <bool MIXED, int IPL, bool MASK, class JB>
+__device__ __forceinline__ void ens_item(const EnsArgs& a, const EnsItem& it, JB* tile)
+{
+    constexpr int UNROLL = IPL == 1 ? 4 : 2;
+    const int lane = threadIdx.x;
+    const double eps2 = a.sys[it.sys].eps2;
+    int ti[IPL];
+    double xi[IPL], yi[IPL], zi[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        ti[q] = it.i0 + q * kLanes + lane;
+        const int ic = q * kLanes + lane < it.icount ? ti[q] : it.i0;
+        xi[q] = a.s.x[ic]; yi[q] = a.s.y[ic]; zi[q] = a.s.z[ic];
+    }
+    double out[IPL][6];
+    if constexpr (MIXED) {
+        float vxi[IPL], vyi[IPL], vzi[IPL];
+#pragma unroll
+        for (int q = 0; q < IPL; ++q) {
+            const int ic = q * kLanes + lane < it.icount ? ti[q] : it.i0;
+            vxi[q] = (float)a.s.vx[ic]; vyi[q] = (float)a.s.vy[ic]; vzi[q] = (float)a.s.vz[ic];
+        }
+        const float e2 = (float)eps2;
+        float hi[IPL][6], lo[IPL][6];
+#pragma unroll
+        for (int q = 0; q < IPL; ++q)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) hi[q][c] = lo[q][c] = 0.f;
+        for (int jt = it.j0; jt < it.j1; jt += it.tile) {
+            const int cnt = min(it.tile, it.j1 - jt);
+            __syncthreads();
+            for (int k = lane; k < cnt; k += kLanes) tile[k] = load_jbmx(a.s, jt + k);
+            __syncthreads();
+            for (int k0 = 0; k0 < cnt; k0 += kFold) {
+                float acc[IPL][6];
+#pragma unroll
+                for (int q = 0; q < IPL; ++q)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) acc[q][c] = 0.f;
+                const int kend = min(cnt, k0 + kFold);
+                if (kend - k0 == kFold) {
+#pragma unroll UNROLL
+                    for (int k = k0; k < k0 + kFold; ++k) {
+                        const JBMX b = tile[k];
+#pragma unroll
+                        for (int q = 0; q < IPL; ++q)
+                            pair_mx<MASK>(b, jt + k, ti[q], xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q], e2, acc[q]);
+                    }
+                } else {
+                    for (int k = k0; k < kend; ++k) {
+                        const JBMX b = tile[k];
+#pragma unroll
+                        for (int q = 0; q < IPL; ++q)
+                            pair_mx<MASK>(b, jt + k, ti[q], xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q], e2, acc[q]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < IPL; ++q)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) two_sum_fold(hi[q][c], lo[q][c], acc[q][c]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IPL; ++q)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[q][c] = (double)hi[q][c] + (double)lo[q][c];
+    } else {
+        double vxi[IPL], vyi[IPL], vzi[IPL];
+#pragma unroll
+        for (int q = 0; q < IPL; ++q) {
+            const int ic = q * kLanes + lane < it.icount ? ti[q] : it.i0;
+            vxi[q] = a.s.vx[ic]; vyi[q] = a.s.vy[ic]; vzi[q] = a.s.vz[ic];
+        }
+        double acc[IPL][6];
+#pragma unroll
+        for (int q = 0; q < IPL; ++q)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[q][c] = 0.0;
+        for (int jt = it.j0; jt < it.j1; jt += it.tile) {
+            const int cnt = min(it.tile, it.j1 - jt);
+            __syncthreads();
+            for (int k = lane; k < cnt; k += kLanes) tile[k] = load_jb64(a.s, jt + k);
+            __syncthreads();
+            if (cnt == it.tile) {
+#pragma unroll UNROLL
+                for (int k = 0; k < cnt; ++k) {
+                    const JB64 b = tile[k];
+#pragma unroll
+                    for (int q = 0; q < IPL; ++q)
+                        pair_f64<MASK>(b, jt + k, ti[q], xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q], eps2, acc[q]);
+                }
+            } else {
+                for (int k = 0; k < cnt; ++k) {
+                    const JB64 b = tile[k];
+#pragma unroll
+                    for (int q = 0; q < IPL; ++q)
+                        pair_f64<MASK>(b, jt + k, ti[q], xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q], eps2, acc[q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IPL; ++q)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[q][c] = acc[q][c];
+    }
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int t = q * kLanes + lane;
+        if (t < it.icount) {
+            if (it.slot < 0) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) a.out.c[c][ti[q]] = out[q][c];
+            } else {
+                double* p = a.partial + (size_t)it.slot * 6 * kMaxTargets + t;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) p[c * kMaxTargets] = out[q][c];
+            }
+        }
+    }
+}
+
 template <bool MIXED, int MAXT, bool MASK>
 __global__ __launch_bounds__(kLanes) void k_ens_force(EnsArgs a)
 {
     using JB = typename std::conditional<MIXED, JBMX, JB64>::type;
     __shared__ JB tile[MAXT];
-    const int lane = threadIdx.x;
     for (;;) {
         const int w = take_item(a.counter);
         if (w >= a.nitems) break;
         const EnsItem it = a.items[w];
-        const double eps2 = a.sys[it.sys].eps2;
-        const int i = it.i0 + lane;
-        const int ic = lane < it.icount ? i : it.i0;
-        const double xi = a.s.x[ic], yi = a.s.y[ic], zi = a.s.z[ic];
-        double out[6];
-        if constexpr (MIXED) {
-            const float vxi = (float)a.s.vx[ic], vyi = (float)a.s.vy[ic], vzi = (float)a.s.vz[ic];
-            const float e2 = (float)eps2;
-            float hi[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, lo[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            for (int jt = it.j0; jt < it.j1; jt += it.tile) {
-                const int cnt = min(it.tile, it.j1 - jt);
-                __syncthreads();
-                for (int k = lane; k < cnt; k += kLanes) tile[k] = load_jbmx(a.s, jt + k);
-                __syncthreads();
-                for (int k0 = 0; k0 < cnt; k0 += kFold) {
-                    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                    const int kend = min(cnt, k0 + kFold);
-                    if (kend - k0 == kFold) {
-#pragma unroll 4
-                        for (int k = k0; k < k0 + kFold; ++k)
-                            pair_mx<MASK>(tile[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, e2, acc);
-                    } else {
-                        for (int k = k0; k < kend; ++k)
-                            pair_mx<MASK>(tile[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, e2, acc);
-                    }
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) two_sum_fold(hi[c], lo[c], acc[c]);
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] = (double)hi[c] + (double)lo[c];
-        } else {
-            const double vxi = a.s.vx[ic], vyi = a.s.vy[ic], vzi = a.s.vz[ic];
-            double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int jt = it.j0; jt < it.j1; jt += it.tile) {
-                const int cnt = min(it.tile, it.j1 - jt);
-                __syncthreads();
-                for (int k = lane; k < cnt; k += kLanes) tile[k] = load_jb64(a.s, jt + k);
-                __syncthreads();
-                if (cnt == it.tile) {
-#pragma unroll 4
-                    for (int k = 0; k < cnt; ++k)
-                        pair_f64<MASK>(tile[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
-                } else {
-                    for (int k = 0; k < cnt; ++k)
-                        pair_f64<MASK>(tile[k], jt + k, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] = acc[c];
-        }
-        if (lane < it.icount) {
-            if (it.slot < 0) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) a.out.c[c][i] = out[c];
-            } else {
-                double* p = a.partial + (size_t)it.slot * 6 * kLanes + lane;
-#pragma unroll
-                for (int c = 0; c < 6; ++c) p[c * kLanes] = out[c];
-            }
-        }
+        if (it.ipl == 2)
+            ens_item<MIXED, 2, MASK>(a, it, tile);
+        else
+            ens_item<MIXED, 1, MASK>(a, it, tile);
     }
 }
 
@@ -161,12 +227,12 @@ __global__ __launch_bounds__(kLanes) void k_ens_reduce(const EnsReduce* r, int n
     const int b = blockIdx.x;
     if (b >= nr) return;
     const EnsReduce q = r[b];
-    const int lane = threadIdx.x;
-    if (lane >= q.icount) return;
-    for (int c = 0; c < 6; ++c) {
-        double v = 0.0;
-        for (int s = 0; s < q.splits; ++s) v += partial[((size_t)(q.slot0 + s) * 6 + c) * kLanes + lane];
-        out.c[c][q.i0 + lane] = v;
+    for (int t = threadIdx.x; t < q.icount; t += kLanes) {
+        for (int c = 0; c < 6; ++c) {
+            double v = 0.0;
+            for (int s = 0; s < q.splits; ++s) v += partial[((size_t)(q.slot0 + s) * 6 + c) * kMaxTargets + t];
+            out.c[c][q.i0 + t] = v;
+        }
     }
 }
 
@@ -250,6 +316,7 @@ struct nbh_ensemble {
     bool any_zero_eps = false;
     int tile[NBH_ENS_BUCKETS] = {128, 128, 128, 128};
     int split[NBH_ENS_BUCKETS] = {1, 1, 1, 1};
+    int ipl[NBH_ENS_BUCKETS] = {1, 1, 1, 1};
 
     // device
     DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot;
@@ -272,8 +339,8 @@ void drop_graph(nbh_ensemble* e)
 }
 
 // Work list for the systems selected by `use` (all when empty), with the
-// per-bucket tile/split of `tile`/`split`.
-void build_items(const nbh_ensemble* e, const int* tile, const int* split,
+// per-bucket tile/split/targets-per-lane of `tile`/`split`/`ipl`.
+void build_items(const nbh_ensemble* e, const int* tile, const int* split, const int* ipl,
                  const std::vector<char>& use, std::vector<EnsItem>& items,
                  std::vector<EnsReduce>& reds, int& nslots)
 {
@@ -285,19 +352,21 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split,
         const int n = e->cnt[k], off = e->off[k];
         const int b = nbh::bucket_of(n);
         const int T = tile[b];
+        const int Q = ipl[b];
+        const int per = nbh::kLanes * Q;
         int S = std::min(split[b], (n + T - 1) / T);
         S = std::max(S, 1);
         const int chunk = (n + S - 1) / S;
         S = (n + chunk - 1) / chunk;
-        for (int ib = 0; ib < n; ib += nbh::kLanes) {
-            const int ic = std::min(nbh::kLanes, n - ib);
+        for (int ib = 0; ib < n; ib += per) {
+            const int ic = std::min(per, n - ib);
             if (S == 1) {
-                items.push_back({k, off + ib, ic, off, off + n, T, -1, 0});
+                items.push_back({k, off + ib, ic, off, off + n, T, -1, Q});
             } else {
                 reds.push_back({off + ib, ic, nslots, S});
                 for (int s = 0; s < S; ++s)
                     items.push_back({k, off + ib, ic, off + s * chunk, off + std::min(n, (s + 1) * chunk), T,
-                                     nslots + s, 0});
+                                     nslots + s, Q});
                 nslots += S;
             }
         }
@@ -313,7 +382,7 @@ int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::
     nbh_ctx* c = e->ctx;
     if (e->items.reserve(sizeof(EnsItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
         e->reduces.reserve(sizeof(EnsReduce) * std::max<size_t>(reds.size(), 1)) != hipSuccess ||
-        e->partial.reserve(sizeof(double) * 6 * nbh::kLanes * std::max(nslots, 1)) != hipSuccess)
+        e->partial.reserve(sizeof(double) * 6 * nbh::kMaxTargets * std::max(nslots, 1)) != hipSuccess)
         return fail(NBH_ERR_ALLOC, "ensemble work-list allocation failed");
     if (!items.empty())
         NBH_HIP(hipMemcpyAsync(e->items.p, items.data(), sizeof(EnsItem) * items.size(), hipMemcpyHostToDevice,
@@ -338,7 +407,7 @@ int rebuild(nbh_ensemble* e)
     std::vector<EnsItem> items;
     std::vector<EnsReduce> reds;
     int nslots = 0;
-    build_items(e, e->tile, e->split, {}, items, reds, nslots);
+    build_items(e, e->tile, e->split, e->ipl, {}, items, reds, nslots);
     int maxt = 64;
     for (const auto& it : items) maxt = std::max(maxt, it.tile);
     drop_graph(e);
@@ -618,42 +687,53 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
             if (nbh::bucket_of(e->cnt[k]) == bk) use[k] = 1, any = true;
         if (!any) continue;
         double best = 1e300;
-        int bt = e->tile[bk], bs = e->split[bk];
-        for (int T : nbh::kTiles) {
-            for (int S : nbh::kSplits) {
-                int tl[NBH_ENS_BUCKETS], sp[NBH_ENS_BUCKETS];
-                std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tl);
-                std::copy(e->split, e->split + NBH_ENS_BUCKETS, sp);
-                tl[bk] = T;
-                sp[bk] = S;
-                std::vector<EnsItem> items;
-                std::vector<EnsReduce> reds;
-                int nslots = 0;
-                build_items(e, tl, sp, use, items, reds, nslots);
-                if ((rc = upload_items(e, items, reds, nslots, T))) break;
-                float t = 0.f;
-                double mn = 1e300;
-                for (int rep = 0; rep < 3 && rc == NBH_OK; ++rep) {
-                    if (hipEventRecord(a, c->stream) != hipSuccess) rc = fail(NBH_ERR_HIP, "event record failed");
-                    if (rc == NBH_OK) rc = enqueue_ens_force(e, nbh::forces_of(e->f1.d(), e->total), false);
-                    if (rc == NBH_OK && hipEventRecord(b, c->stream) != hipSuccess)
-                        rc = fail(NBH_ERR_HIP, "event record failed");
-                    if (rc == NBH_OK && hipEventSynchronize(b) != hipSuccess) rc = fail(NBH_ERR_HIP, "sync failed");
-                    if (rc == NBH_OK && hipEventElapsedTime(&t, a, b) != hipSuccess)
-                        rc = fail(NBH_ERR_HIP, "elapsed time failed");
-                    if (rep > 0) mn = std::min(mn, (double)t);  // first run warms up
+        int bt = e->tile[bk], bs = e->split[bk], bq = e->ipl[bk];
+        int nmax = 0;
+        for (int k = 0; k < e->nsys; ++k)
+            if (use[k]) nmax = std::max(nmax, e->cnt[k]);
+        for (int Q : {1, 2}) {
+            if (Q == 2 && nmax <= nbh::kLanes) continue;  // one lane-set already holds every target
+            for (int T : nbh::kTiles) {
+                for (int S : nbh::kSplits) {
+                    int tl[NBH_ENS_BUCKETS], sp[NBH_ENS_BUCKETS], ql[NBH_ENS_BUCKETS];
+                    std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tl);
+                    std::copy(e->split, e->split + NBH_ENS_BUCKETS, sp);
+                    std::copy(e->ipl, e->ipl + NBH_ENS_BUCKETS, ql);
+                    tl[bk] = T;
+                    sp[bk] = S;
+                    ql[bk] = Q;
+                    std::vector<EnsItem> items;
+                    std::vector<EnsReduce> reds;
+                    int nslots = 0;
+                    build_items(e, tl, sp, ql, use, items, reds, nslots);
+                    if ((rc = upload_items(e, items, reds, nslots, T))) break;
+                    float t = 0.f;
+                    double mn = 1e300;
+                    for (int rep = 0; rep < 3 && rc == NBH_OK; ++rep) {
+                        if (hipEventRecord(a, c->stream) != hipSuccess) rc = fail(NBH_ERR_HIP, "event record failed");
+                        if (rc == NBH_OK) rc = enqueue_ens_force(e, nbh::forces_of(e->f1.d(), e->total), false);
+                        if (rc == NBH_OK && hipEventRecord(b, c->stream) != hipSuccess)
+                            rc = fail(NBH_ERR_HIP, "event record failed");
+                        if (rc == NBH_OK && hipEventSynchronize(b) != hipSuccess) rc = fail(NBH_ERR_HIP, "sync failed");
+                        if (rc == NBH_OK && hipEventElapsedTime(&t, a, b) != hipSuccess)
+                            rc = fail(NBH_ERR_HIP, "elapsed time failed");
+                        if (rep > 0) mn = std::min(mn, (double)t);  // first run warms up
+                    }
+                    if (rc) break;
+                    if (mn < best) {
+                        best = mn;
+                        bt = T;
+                        bs = S;
+                        bq = Q;
+                    }
                 }
                 if (rc) break;
-                if (mn < best) {
-                    best = mn;
-                    bt = T;
-                    bs = S;
-                }
             }
             if (rc) break;
         }
         e->tile[bk] = bt;
         e->split[bk] = bs;
+        e->ipl[bk] = bq;
         best_ms[bk] = best;
     }
     c->profiling = prof;
@@ -665,6 +745,27 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
     if (tiles) std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tiles);
     if (jsplits) std::copy(e->split, e->split + NBH_ENS_BUCKETS, jsplits);
     if (ms_out) std::copy(best_ms, best_ms + NBH_ENS_BUCKETS, ms_out);
+    return NBH_OK;
+}
+
+int nbh_ens_set_targets_per_lane(nbh_ensemble* e, const int* ipls)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (!ipls) return fail(NBH_ERR_ARG, "ipls is NULL");
+    for (int b = 0; b < NBH_ENS_BUCKETS; ++b)
+        if (ipls[b] != 1 && ipls[b] != 2)
+            return fail(NBH_ERR_ARG, "bucket %d: targets per lane must be 1 or 2, got %d", b, ipls[b]);
+    NBH_HIP(hipSetDevice(e->ctx->cfg.device));
+    std::copy(ipls, ipls + NBH_ENS_BUCKETS, e->ipl);
+    return rebuild(e);
+}
+
+int nbh_ens_tuning(nbh_ensemble* e, int* tiles, int* jsplits, int* ipls)
+{
+    if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
+    if (tiles) std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tiles);
+    if (jsplits) std::copy(e->split, e->split + NBH_ENS_BUCKETS, jsplits);
+    if (ipls) std::copy(e->ipl, e->ipl + NBH_ENS_BUCKETS, ipls);
     return NBH_OK;
 }
 

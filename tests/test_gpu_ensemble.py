@@ -63,15 +63,34 @@ def test_initial_forces_mixed(nb, oracle, small):
         check_systems(e, f, state, oracle, eps2, mixed=True)
 
 
+@pytest.mark.parametrize("ipl", [1, 2])
 @pytest.mark.parametrize("tile", [64, 128, 256])
 @pytest.mark.parametrize("split", [1, 2, 4, 8])
-def test_tunings_agree(nb, oracle, small, tile, split):
+def test_tunings_agree(nb, oracle, small, tile, split, ipl):
     counts, eps2, state = small
     with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
         e.set_tuning([tile] * 4, [split] * 4)
+        e.set_targets_per_lane([ipl] * 4)
+        assert e.tuning() == {"tile": [tile] * 4, "jsplit": [split] * 4, "ipl": [ipl] * 4}
         e.upload(state)
         _, f = e.download()
         check_systems(e, f, state, oracle, eps2)
+
+
+@pytest.mark.parametrize("split", [1, 4])
+def test_targets_per_lane_do_not_change_sums(nb, small, split):
+    """Two targets per lane share each source record but keep each target's
+    summation order, so FP64 and mixed results are bit-identical to one."""
+    counts, eps2, state = small
+    for prec in ("fp64", "mixed"):
+        outs = []
+        with nb.Backend(prec) as b, nb.Ensemble(b, counts, eps2) as e:
+            e.set_tuning([128] * 4, [split] * 4)
+            for ipl in (1, 2, 1):
+                e.set_targets_per_lane([ipl] * 4)
+                e.upload(state)
+                outs.append(e.download()[1])
+        assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
 def test_tile_length_does_not_change_sums(nb, small):
@@ -93,6 +112,7 @@ def test_steps_vs_oracle(nb, oracle, small):
     dt, nsteps = 1 / 256, 4
     with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
         e.set_tuning([64, 128, 128, 256], [1, 2, 2, 4])
+        e.set_targets_per_lane([1, 2, 1, 2])
         e.upload(state)
         e.set_dt(dt)
         e.step(nsteps)
@@ -174,4 +194,7 @@ def test_bad_arguments(nb):
             assert ex.value.code == nb.ERR_STATE
             with pytest.raises(nb.NBodyError) as ex:
                 e.set_tuning([96, 64, 64, 64], [1, 1, 1, 1])
+            assert ex.value.code == nb.ERR_ARG
+            with pytest.raises(nb.NBodyError) as ex:
+                e.set_targets_per_lane([1, 3, 1, 1])
             assert ex.value.code == nb.ERR_ARG
