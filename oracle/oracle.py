@@ -176,6 +176,8 @@ class Reference:
         L.ref_eps2.restype = C.c_double
         L.ref_force.restype = None
         L.ref_force.argtypes = [C.c_int] + [_D] * 13
+        L.ref_force_threads.restype = None
+        L.ref_force_threads.argtypes = [C.c_int, C.c_int] + [_D] * 13
         L.ref_kinetic.restype = C.c_double
         L.ref_kinetic.argtypes = [C.c_int] + [_D] * 4
         L.ref_potential.restype = C.c_double
@@ -192,11 +194,17 @@ class Reference:
     def invsqrt2d(self, x):
         return self.lib.ref_invsqrt2d(float(x))
 
-    def force(self, state):
+    def force(self, state, nthreads=1):
+        """force_calculation of the compiled reference; nthreads > 1 runs its
+        OpenMP loop threaded (results then depend on the merge order)."""
         s = _f64(state).copy()
         n = s.shape[1]
         out = np.zeros((6, n))
-        self.lib.ref_force(n, *[_p(s[k]) for k in range(7)], *[_p(out[k]) for k in range(6)])
+        if nthreads == 1:
+            self.lib.ref_force(n, *[_p(s[k]) for k in range(7)], *[_p(out[k]) for k in range(6)])
+        else:
+            self.lib.ref_force_threads(n, int(nthreads), *[_p(s[k]) for k in range(7)],
+                                       *[_p(out[k]) for k in range(6)])
         return out
 
     def kinetic(self, state):

@@ -38,6 +38,23 @@ double ref_invsqrt2d(double x) { return InvSqrt2D(x); }
 
 double ref_eps2() { return EPS2; }
 
+// force_calculation (include/kernel.h:15-19) on one rank, with nthreads
+// OpenMP threads (1: the reproducible order; more: the CPU timing baseline).
+void ref_force_threads(int n, int nthreads, double* x, double* y, double* z, double* w, double* vx,
+                       double* vy, double* vz, double* ax, double* ay, double* az,
+                       double* jx, double* jy, double* jz)
+{
+    omp_set_num_threads(nthreads > 0 ? nthreads : 1);
+    Sys sys;
+    sys.N = n;
+    sys.rank = 0;
+    sys.size = 1;
+    sys.start = 0;
+    sys.end = n;
+    force_calculation(x, y, z, w, vx, vy, vz, ax, ay, az, jx, jy, jz, sys);
+    omp_set_num_threads(1);
+}
+
 // force_calculation (include/kernel.h:15-19) on one rank.
 void ref_force(int n, double* x, double* y, double* z, double* w, double* vx,
                double* vy, double* vz, double* ax, double* ay, double* az,

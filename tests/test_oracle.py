@@ -262,3 +262,14 @@ def test_multithreaded_oracle_close_to_serial(oracle, nb):
     a = oracle.force(s, 1e-4, nthreads=1)
     b = oracle.force(s, 1e-4, nthreads=4)
     assert body_relerr(b[:3], a[:3]).max() < 1e-13
+
+
+def test_reference_threaded_force(reference):
+    """The compiled reference's threaded force (the CPU timing baseline)
+    matches its single-thread result to the rounding of the thread merge."""
+    r64, _ = reference
+    s = np.random.default_rng(5).standard_normal((7, 2048))
+    s[3] = 1.0 / 2048
+    one = r64.force(s)
+    many = r64.force(s, nthreads=4)
+    assert np.abs(many - one).max() <= 1e-13 * np.abs(one).max()
