@@ -64,7 +64,9 @@ typedef struct nbh_config {
                        64, 128 or 256; 0 = pick by size */
     int jsplit;     /* j-range split across workgroups; 0 = pick by size */
     int ipl;        /* targets per lane, 1 or 2; 0 = pick by size */
-    int pairing;    /* enum nbh_pairing; 0 = NBH_PAIR_DIRECTED */
+    int pairing;    /* enum nbh_pairing; 0 = NBH_PAIR_DIRECTED.  block,
+                       jsplit and ipl shape the directed kernels; the
+                       symmetric kernels have a fixed shape */
 } nbh_config;
 
 /* Description of the last failure on this thread ("" if none). */
