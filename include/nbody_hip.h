@@ -197,9 +197,9 @@ int nbh_device_info(int device, int *compute_units, int *wave_size,
  * arrays are the systems' arrays concatenated in system order.  The force
  * evaluation is one persistent launch: one-wave workgroups take work items
  * (system, block of 64 or 128 targets, source range) from a device queue,
- * largest first; a size bucket's work items use the LDS tile length, j-split
- * and targets per lane chosen for that bucket (by nbh_ens_autotune or
- * nbh_ens_set_tuning / nbh_ens_set_targets_per_lane).
+ * largest first; a size bucket's work items use the LDS tile length and
+ * j-split chosen for that bucket, and all items one or two targets per lane
+ * (by nbh_ens_autotune or nbh_ens_set_tuning / nbh_ens_set_targets_per_lane).
  * Precision and device come from the context.  The step is the shared-dt
  * Hermite step of nbh_step applied to every system. */
 
@@ -229,18 +229,19 @@ int nbh_ens_download(nbh_ensemble *ens, double *x, double *y, double *z,
                      double *jy, double *jz);
 /* Per-system kinetic and softened potential energy (FP64), arrays [nsys]. */
 int nbh_ens_energy(nbh_ensemble *ens, double *kinetic, double *potential);
-/* Times the candidate (targets per lane in {1,2}) x (tile in {64,128,256})
- * x (jsplit in {1,2,4,8}) for every populated bucket on the device and keeps
- * the fastest.  Outputs [NBH_ENS_BUCKETS] each, may be NULL; ms = best force
- * time of the bucket alone (0 for empty buckets). */
+/* Times the candidate (tile in {64,128,256}) x (jsplit in {1,2,4,8}) for
+ * every populated bucket on the device and keeps the fastest, then one or
+ * two targets per lane for the whole work list.  Outputs [NBH_ENS_BUCKETS]
+ * each, may be NULL; ms = best force time of the bucket alone (0 for empty
+ * buckets). */
 int nbh_ens_autotune(nbh_ensemble *ens, int *tiles, int *jsplits, double *ms);
 /* Sets the per-bucket tile (64, 128 or 256) and jsplit (>= 1) directly. */
 int nbh_ens_set_tuning(nbh_ensemble *ens, const int *tiles, const int *jsplits);
-/* Sets the per-bucket targets per lane (1 or 2) directly. */
-int nbh_ens_set_targets_per_lane(nbh_ensemble *ens, const int *ipls);
-/* Current per-bucket tile, jsplit and targets per lane ([NBH_ENS_BUCKETS]
- * each, may be NULL). */
-int nbh_ens_tuning(nbh_ensemble *ens, int *tiles, int *jsplits, int *ipls);
+/* Sets the targets per lane (1 or 2) of every work item. */
+int nbh_ens_set_targets_per_lane(nbh_ensemble *ens, int ipl);
+/* Current per-bucket tile and jsplit ([NBH_ENS_BUCKETS] each) and targets
+ * per lane; any pointer may be NULL. */
+int nbh_ens_tuning(nbh_ensemble *ens, int *tiles, int *jsplits, int *ipl);
 /* Work items per force evaluation, persistent workgroups, largest tile. */
 int nbh_ens_info(nbh_ensemble *ens, int *items, int *workgroups, int *max_tile);
 

@@ -6,8 +6,9 @@
 //
 // Force evaluation: one persistent launch of one-wave (64-lane) workgroups.
 // Each wave takes work items from a device queue with an atomic counter:
-// item = (system, 64 or 128 consecutive targets (one or two per lane), a
-// source range, an LDS tile length, an output slot).  Items are sorted by cost, largest first, so the
+// item = (system, 64 or 128 consecutive targets (one or two per lane, the
+// same for every item of a list), a source range, an LDS tile length, an
+// output slot).  Items are sorted by cost, largest first, so the
 // long items start early and the short ones fill the tail.  Systems whose
 // bucket uses a j-split write per-item partial sums into slots that a second
 // kernel adds in split order (deterministic).  Every wave leaves the loop
@@ -46,7 +47,7 @@ struct EnsItem {
     int j0, j1;  // source range (global indices)
     int tile;    // LDS tile length for this item
     int slot;    // partial-sum slot, or -1: write the forces directly
-    int ipl;     // targets per lane: lane t holds i0 + t and, if 2, i0 + 64 + t
+    int ipl;     // targets per lane of the list: lane t holds i0 + t and, if 2, i0 + 64 + t
 };
 
 struct EnsReduce {
@@ -205,7 +206,9 @@ __device__ __forceinline__ void ens_item(const EnsArgs& a, const EnsItem& it, JB
     }
 }
 
-template <bool MIXED, int MAXT, bool MASK>
+// One instance per targets-per-lane count, so that the one-target kernel
+// keeps its lower register count (and occupancy).
+template <bool MIXED, int MAXT, bool MASK, int IPL>
 __global__ __launch_bounds__(kLanes) void k_ens_force(EnsArgs a)
 {
     using JB = typename std::conditional<MIXED, JBMX, JB64>::type;
@@ -213,11 +216,7 @@ __global__ __launch_bounds__(kLanes) void k_ens_force(EnsArgs a)
     for (;;) {
         const int w = take_item(a.counter);
         if (w >= a.nitems) break;
-        const EnsItem it = a.items[w];
-        if (it.ipl == 2)
-            ens_item<MIXED, 2, MASK>(a, it, tile);
-        else
-            ens_item<MIXED, 1, MASK>(a, it, tile);
+        ens_item<MIXED, IPL, MASK>(a, a.items[w], tile);
     }
 }
 
@@ -281,16 +280,24 @@ __global__ __launch_bounds__(kEnBlock) void k_ens_energy(State s, const EnsSys* 
     }
 }
 
-template <bool MIXED, int MAXT, bool MASK>
+template <bool MIXED, int MAXT, bool MASK, int IPL>
 const void* ens_kernel_ptr()
 {
-    return reinterpret_cast<const void*>(&k_ens_force<MIXED, MAXT, MASK>);
+    return reinterpret_cast<const void*>(&k_ens_force<MIXED, MAXT, MASK, IPL>);
 }
 
-const void* pick_ens_kernel(bool mixed, int maxt, bool mask)
+template <bool MIXED, int MAXT>
+const void* ens_kernel_for(bool mask, int ipl)
 {
-#define NBH_ENS_K(M, T)                                                                  \
-    if (mixed == M && maxt == T) return mask ? ens_kernel_ptr<M, T, true>() : ens_kernel_ptr<M, T, false>();
+    if (ipl == 2) return mask ? ens_kernel_ptr<MIXED, MAXT, true, 2>() : ens_kernel_ptr<MIXED, MAXT, false, 2>();
+    return mask ? ens_kernel_ptr<MIXED, MAXT, true, 1>() : ens_kernel_ptr<MIXED, MAXT, false, 1>();
+}
+
+const void* pick_ens_kernel(bool mixed, int maxt, bool mask, int ipl)
+{
+    if (ipl != 1 && ipl != 2) return nullptr;
+#define NBH_ENS_K(M, T) \
+    if (mixed == M && maxt == T) return ens_kernel_for<M, T>(mask, ipl);
     NBH_ENS_K(false, 64)
     NBH_ENS_K(false, 128)
     NBH_ENS_K(false, 256)
@@ -316,11 +323,12 @@ struct nbh_ensemble {
     bool any_zero_eps = false;
     int tile[NBH_ENS_BUCKETS] = {128, 128, 128, 128};
     int split[NBH_ENS_BUCKETS] = {1, 1, 1, 1};
-    int ipl[NBH_ENS_BUCKETS] = {1, 1, 1, 1};
+    int ipl = 1;  // targets per lane, every work item
 
     // device
     DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot;
     int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0;
+    int list_ipl = 1;  // targets per lane of the uploaded work list
     bool uploaded = false, dt_set = false;
     hipGraphExec_t gexec = nullptr;
     int g_steps = 0;
@@ -339,8 +347,8 @@ void drop_graph(nbh_ensemble* e)
 }
 
 // Work list for the systems selected by `use` (all when empty), with the
-// per-bucket tile/split/targets-per-lane of `tile`/`split`/`ipl`.
-void build_items(const nbh_ensemble* e, const int* tile, const int* split, const int* ipl,
+// per-bucket tile/split of `tile`/`split` and `ipl` targets per lane.
+void build_items(const nbh_ensemble* e, const int* tile, const int* split, int ipl,
                  const std::vector<char>& use, std::vector<EnsItem>& items,
                  std::vector<EnsReduce>& reds, int& nslots)
 {
@@ -352,7 +360,7 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, const
         const int n = e->cnt[k], off = e->off[k];
         const int b = nbh::bucket_of(n);
         const int T = tile[b];
-        const int Q = ipl[b];
+        const int Q = ipl;
         const int per = nbh::kLanes * Q;
         int S = std::min(split[b], (n + T - 1) / T);
         S = std::max(S, 1);
@@ -377,7 +385,7 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, const
 }
 
 int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::vector<EnsReduce>& reds,
-                 int nslots, int maxtile)
+                 int nslots, int maxtile, int ipl)
 {
     nbh_ctx* c = e->ctx;
     if (e->items.reserve(sizeof(EnsItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
@@ -395,7 +403,9 @@ int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::
     e->nreduce = (int)reds.size();
     e->nslots = nslots;
     e->maxtile = maxtile;
-    const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, maxtile, e->any_zero_eps);
+    e->list_ipl = ipl;
+    const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, maxtile, e->any_zero_eps, ipl);
+    if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d, %d targets per lane", maxtile, ipl);
     int per_cu = 0;
     NBH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, nbh::kLanes, 0));
     e->grid = std::max(1, std::min(e->nitems, per_cu * c->cus));
@@ -411,7 +421,7 @@ int rebuild(nbh_ensemble* e)
     int maxt = 64;
     for (const auto& it : items) maxt = std::max(maxt, it.tile);
     drop_graph(e);
-    return upload_items(e, items, reds, nslots, maxt);
+    return upload_items(e, items, reds, nslots, maxt, e->ipl);
 }
 
 // Force evaluation of the current work list: state -> out.
@@ -435,7 +445,7 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
         if (rc) return rc;
         NBH_HIP(hipEventRecord(ep.a, c->stream));
     }
-    const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, e->maxtile, e->any_zero_eps);
+    const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, e->maxtile, e->any_zero_eps, e->list_ipl);
     if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d", e->maxtile);
     void* args[] = {&a};
     NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, c->stream));
@@ -680,6 +690,23 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
     int rc = NBH_OK;
     const bool prof = c->profiling;
     c->profiling = false;
+    // min over two timed force evaluations (after one untimed) of the uploaded list
+    auto time_list = [&](double* out_ms) -> int {
+        float t = 0.f;
+        double mn = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {
+            if (hipEventRecord(a, c->stream) != hipSuccess) return fail(NBH_ERR_HIP, "event record failed");
+            int r = enqueue_ens_force(e, nbh::forces_of(e->f1.d(), e->total), false);
+            if (r) return r;
+            if (hipEventRecord(b, c->stream) != hipSuccess) return fail(NBH_ERR_HIP, "event record failed");
+            if (hipEventSynchronize(b) != hipSuccess) return fail(NBH_ERR_HIP, "sync failed");
+            if (hipEventElapsedTime(&t, a, b) != hipSuccess) return fail(NBH_ERR_HIP, "elapsed time failed");
+            if (rep > 0) mn = std::min(mn, (double)t);  // first run warms up
+        }
+        *out_ms = mn;
+        return (int)NBH_OK;
+    };
+    // tile and split per bucket, that bucket's systems alone
     for (int bk = 0; bk < NBH_ENS_BUCKETS && rc == NBH_OK; ++bk) {
         std::vector<char> use(e->nsys, 0);
         bool any = false;
@@ -687,54 +714,46 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
             if (nbh::bucket_of(e->cnt[k]) == bk) use[k] = 1, any = true;
         if (!any) continue;
         double best = 1e300;
-        int bt = e->tile[bk], bs = e->split[bk], bq = e->ipl[bk];
-        int nmax = 0;
-        for (int k = 0; k < e->nsys; ++k)
-            if (use[k]) nmax = std::max(nmax, e->cnt[k]);
-        for (int Q : {1, 2}) {
-            if (Q == 2 && nmax <= nbh::kLanes) continue;  // one lane-set already holds every target
-            for (int T : nbh::kTiles) {
-                for (int S : nbh::kSplits) {
-                    int tl[NBH_ENS_BUCKETS], sp[NBH_ENS_BUCKETS], ql[NBH_ENS_BUCKETS];
-                    std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tl);
-                    std::copy(e->split, e->split + NBH_ENS_BUCKETS, sp);
-                    std::copy(e->ipl, e->ipl + NBH_ENS_BUCKETS, ql);
-                    tl[bk] = T;
-                    sp[bk] = S;
-                    ql[bk] = Q;
-                    std::vector<EnsItem> items;
-                    std::vector<EnsReduce> reds;
-                    int nslots = 0;
-                    build_items(e, tl, sp, ql, use, items, reds, nslots);
-                    if ((rc = upload_items(e, items, reds, nslots, T))) break;
-                    float t = 0.f;
-                    double mn = 1e300;
-                    for (int rep = 0; rep < 3 && rc == NBH_OK; ++rep) {
-                        if (hipEventRecord(a, c->stream) != hipSuccess) rc = fail(NBH_ERR_HIP, "event record failed");
-                        if (rc == NBH_OK) rc = enqueue_ens_force(e, nbh::forces_of(e->f1.d(), e->total), false);
-                        if (rc == NBH_OK && hipEventRecord(b, c->stream) != hipSuccess)
-                            rc = fail(NBH_ERR_HIP, "event record failed");
-                        if (rc == NBH_OK && hipEventSynchronize(b) != hipSuccess) rc = fail(NBH_ERR_HIP, "sync failed");
-                        if (rc == NBH_OK && hipEventElapsedTime(&t, a, b) != hipSuccess)
-                            rc = fail(NBH_ERR_HIP, "elapsed time failed");
-                        if (rep > 0) mn = std::min(mn, (double)t);  // first run warms up
-                    }
-                    if (rc) break;
-                    if (mn < best) {
-                        best = mn;
-                        bt = T;
-                        bs = S;
-                        bq = Q;
-                    }
+        int bt = e->tile[bk], bs = e->split[bk];
+        for (int T : nbh::kTiles) {
+            for (int S : nbh::kSplits) {
+                int tl[NBH_ENS_BUCKETS], sp[NBH_ENS_BUCKETS];
+                std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tl);
+                std::copy(e->split, e->split + NBH_ENS_BUCKETS, sp);
+                tl[bk] = T;
+                sp[bk] = S;
+                std::vector<EnsItem> items;
+                std::vector<EnsReduce> reds;
+                int nslots = 0;
+                build_items(e, tl, sp, e->ipl, use, items, reds, nslots);
+                double ms = 0.0;
+                if ((rc = upload_items(e, items, reds, nslots, T, e->ipl)) || (rc = time_list(&ms))) break;
+                if (ms < best) {
+                    best = ms;
+                    bt = T;
+                    bs = S;
                 }
-                if (rc) break;
             }
             if (rc) break;
         }
         e->tile[bk] = bt;
         e->split[bk] = bs;
-        e->ipl[bk] = bq;
         best_ms[bk] = best;
+    }
+    // then the targets per lane of the whole list
+    if (rc == NBH_OK) {
+        double best = 1e300;
+        int bq = e->ipl;
+        for (int Q : {1, 2}) {
+            e->ipl = Q;
+            double ms = 0.0;
+            if ((rc = rebuild(e)) || (rc = time_list(&ms))) break;
+            if (ms < best) {
+                best = ms;
+                bq = Q;
+            }
+        }
+        e->ipl = bq;
     }
     c->profiling = prof;
     (void)hipEventDestroy(a);
@@ -748,24 +767,21 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
     return NBH_OK;
 }
 
-int nbh_ens_set_targets_per_lane(nbh_ensemble* e, const int* ipls)
+int nbh_ens_set_targets_per_lane(nbh_ensemble* e, int ipl)
 {
     if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
-    if (!ipls) return fail(NBH_ERR_ARG, "ipls is NULL");
-    for (int b = 0; b < NBH_ENS_BUCKETS; ++b)
-        if (ipls[b] != 1 && ipls[b] != 2)
-            return fail(NBH_ERR_ARG, "bucket %d: targets per lane must be 1 or 2, got %d", b, ipls[b]);
+    if (ipl != 1 && ipl != 2) return fail(NBH_ERR_ARG, "targets per lane must be 1 or 2, got %d", ipl);
     NBH_HIP(hipSetDevice(e->ctx->cfg.device));
-    std::copy(ipls, ipls + NBH_ENS_BUCKETS, e->ipl);
+    e->ipl = ipl;
     return rebuild(e);
 }
 
-int nbh_ens_tuning(nbh_ensemble* e, int* tiles, int* jsplits, int* ipls)
+int nbh_ens_tuning(nbh_ensemble* e, int* tiles, int* jsplits, int* ipl)
 {
     if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
     if (tiles) std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tiles);
     if (jsplits) std::copy(e->split, e->split + NBH_ENS_BUCKETS, jsplits);
-    if (ipls) std::copy(e->ipl, e->ipl + NBH_ENS_BUCKETS, ipls);
+    if (ipl) *ipl = e->ipl;
     return NBH_OK;
 }
 

@@ -92,8 +92,8 @@ def lib():
     L.nbh_ens_energy.argtypes = [vp, _D, _D]
     L.nbh_ens_autotune.argtypes = [vp, _I, _I, _D]
     L.nbh_ens_set_tuning.argtypes = [vp, _I, _I]
-    L.nbh_ens_set_targets_per_lane.argtypes = [vp, _I]
-    L.nbh_ens_tuning.argtypes = [vp, _I, _I, _I]
+    L.nbh_ens_set_targets_per_lane.argtypes = [vp, C.c_int]
+    L.nbh_ens_tuning.argtypes = [vp, _I, _I, C.POINTER(C.c_int)]
     L.nbh_ens_info.argtypes = [vp, _I, _I, _I]
     L.nbh_reference_ic.argtypes = [C.c_int] + [C.c_double] * 4 + [_D] * 7
     L.nbh_force_vjp.argtypes = [vp, C.c_int, _D, _D, _D]
@@ -400,13 +400,14 @@ class Ensemble:
         return {**self.tuning(), "bucket_ms": ms.tolist()}
 
     def tuning(self):
-        t, sp, q = (np.zeros(ENS_BUCKETS, dtype=np.int32) for _ in range(3))
-        _check(lib().nbh_ens_tuning(self._h, *[a.ctypes.data_as(C.POINTER(C.c_int)) for a in (t, sp, q)]))
-        return {"tile": t.tolist(), "jsplit": sp.tolist(), "ipl": q.tolist()}
+        t, sp = np.zeros(ENS_BUCKETS, dtype=np.int32), np.zeros(ENS_BUCKETS, dtype=np.int32)
+        q = C.c_int()
+        _check(lib().nbh_ens_tuning(self._h, t.ctypes.data_as(C.POINTER(C.c_int)),
+                                    sp.ctypes.data_as(C.POINTER(C.c_int)), C.byref(q)))
+        return {"tile": t.tolist(), "jsplit": sp.tolist(), "ipl": q.value}
 
-    def set_targets_per_lane(self, ipls):
-        q = np.ascontiguousarray(ipls, dtype=np.int32)
-        _check(lib().nbh_ens_set_targets_per_lane(self._h, q.ctypes.data_as(C.POINTER(C.c_int))))
+    def set_targets_per_lane(self, ipl):
+        _check(lib().nbh_ens_set_targets_per_lane(self._h, int(ipl)))
 
     def set_tuning(self, tiles, jsplits):
         t = np.ascontiguousarray(tiles, dtype=np.int32)

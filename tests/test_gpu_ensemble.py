@@ -70,8 +70,8 @@ def test_tunings_agree(nb, oracle, small, tile, split, ipl):
     counts, eps2, state = small
     with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
         e.set_tuning([tile] * 4, [split] * 4)
-        e.set_targets_per_lane([ipl] * 4)
-        assert e.tuning() == {"tile": [tile] * 4, "jsplit": [split] * 4, "ipl": [ipl] * 4}
+        e.set_targets_per_lane(ipl)
+        assert e.tuning() == {"tile": [tile] * 4, "jsplit": [split] * 4, "ipl": ipl}
         e.upload(state)
         _, f = e.download()
         check_systems(e, f, state, oracle, eps2)
@@ -87,7 +87,7 @@ def test_targets_per_lane_do_not_change_sums(nb, small, split):
         with nb.Backend(prec) as b, nb.Ensemble(b, counts, eps2) as e:
             e.set_tuning([128] * 4, [split] * 4)
             for ipl in (1, 2, 1):
-                e.set_targets_per_lane([ipl] * 4)
+                e.set_targets_per_lane(ipl)
                 e.upload(state)
                 outs.append(e.download()[1])
         assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
@@ -112,7 +112,7 @@ def test_steps_vs_oracle(nb, oracle, small):
     dt, nsteps = 1 / 256, 4
     with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
         e.set_tuning([64, 128, 128, 256], [1, 2, 2, 4])
-        e.set_targets_per_lane([1, 2, 1, 2])
+        e.set_targets_per_lane(2)
         e.upload(state)
         e.set_dt(dt)
         e.step(nsteps)
@@ -196,5 +196,5 @@ def test_bad_arguments(nb):
                 e.set_tuning([96, 64, 64, 64], [1, 1, 1, 1])
             assert ex.value.code == nb.ERR_ARG
             with pytest.raises(nb.NBodyError) as ex:
-                e.set_targets_per_lane([1, 3, 1, 1])
+                e.set_targets_per_lane(3)
             assert ex.value.code == nb.ERR_ARG
