@@ -73,6 +73,7 @@ __device__ __forceinline__ double row_rot(double v)
 // there on; per-tile FP32 target sums folded into a two-sum (hi, lo) pair.
 template <int K>
 struct SymMixed {
+    static constexpr int kWaves = 5;
     using Rec = JBMX;
     using Sum = float;
     using Eps = float;
@@ -160,6 +161,7 @@ struct SymMixed {
 // added; per-tile sums added to FP64 totals.
 template <int K>
 struct SymF64 {
+    static constexpr int kWaves = 3;
     using Rec = JB64;
     using Sum = double;
     using Eps = double;
@@ -256,8 +258,13 @@ struct SymArgs {
     Forces out;   // initialised by the block-diagonal pass
 };
 
+// Resident waves per SIMD asked of the compiler: T::kWaves (mixed 5: 96
+// VGPRs, a few spills; FP64 3).  gfx950 issues FP32 at twice the one-wave
+// rate only by pairing instructions of two waves, so more resident waves
+// raise the dual-issue share (mixed: +1.6% at n = 2^20 over 4 waves).
 template <template <int> class TT, int K, int W>
-__global__ __launch_bounds__(64 * W) void k_force_sym(SymArgs a)
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::kWaves, 8))) void k_force_sym(
+    SymArgs a)
 {
     using T = TT<K>;
     using Sum = typename T::Sum;
@@ -302,7 +309,6 @@ __global__ __launch_bounds__(64 * W) void k_force_sym(SymArgs a)
     for (int tt = 0; tt < ntiles; ++tt) {
         const int buf = tt & 1;
         const bool more = tt + 1 < ntiles;
-        if (more && t < kSymTile) st = T::load_rec(a.s, tile_src(tt + 1) + t);
         Sum ai[K][6];
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -324,7 +330,10 @@ __global__ __launch_bounds__(64 * W) void k_force_sym(SymArgs a)
             }
         }
         T::fold(tot, ai);
+        // the next tile is loaded after the arithmetic, not held in registers
+        // across it (12 VGPRs); the other resident workgroups cover the load
         if (more && t < kSymTile) {
+            st = T::load_rec(a.s, tile_src(tt + 1) + t);
             dup[buf ^ 1][t >> 4][t & 15] = st;
             dup[buf ^ 1][t >> 4][(t & 15) + 16] = st;
         }
