@@ -46,8 +46,8 @@ enum nbh_precision {
  * pair once, applied to both bodies (Newton's third law, as the reference's
  * j < i loops at src/kernel.cpp:1126,1187), sums combined with FP64 atomic
  * adds, reproducible to FP64 rounding only.  SYMMETRIC is used (in both
- * precisions) when n is a multiple of 512 and at least 1024; other n run
- * DIRECTED (nbh_launch_pairing reports which). */
+ * precisions) when n >= 1024; smaller n run DIRECTED (nbh_launch_pairing
+ * reports which). */
 enum nbh_pairing {
     NBH_PAIR_DIRECTED = 0,
     NBH_PAIR_SYMMETRIC = 1

@@ -46,7 +46,7 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
 
 // Directed pair kernel restricted to the diagonal blocks: target i gets the
 // sources of its own block [B*(i/B), B*(i/B)+B), self-pair excluded; the
-// outputs are overwritten.  n % B == 0 and B % 256 == 0.
+// outputs are overwritten (the last block may be partial).  B % 256 == 0.
 hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
                                   const Forces& out, hipStream_t st);
 

@@ -524,7 +524,7 @@ hipError_t launch_force_blockdiag(int precision, const State& s, int n, double e
                                   const Forces& out, hipStream_t st)
 {
     constexpr int BLOCK = 256;
-    if (B <= 0 || B % BLOCK != 0 || n % B != 0) return hipErrorInvalidValue;
+    if (B <= 0 || B % BLOCK != 0) return hipErrorInvalidValue;
     PairArgs a;
     a.s = s;
     a.n = n;

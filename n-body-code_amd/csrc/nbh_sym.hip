@@ -7,7 +7,9 @@
 //
 // Layout of the work:
 //   * bodies are cut into blocks of B = 64*K*W (K targets per lane, W waves
-//     per workgroup); block P's pairs with block Q != P are evaluated by one
+//     per workgroup; the last block may be partial, its missing lanes padded
+//     with massless bodies far away); block P's pairs with block Q != P are
+//     evaluated by one
 //     workgroup, for Q = P+1 .. P+H (mod nb), H = (nb-1)/2, plus Q = P+nb/2
 //     for P < nb/2 when nb is even, so every unordered block pair is taken
 //     exactly once and every block has the same work;
@@ -45,6 +47,13 @@ namespace nbh {
 namespace {
 
 constexpr int kSymTile = 64;  // sources per LDS tile (4 groups of 16)
+
+// Lanes past the last body (the partial last block) hold pad bodies: zero
+// mass, at rest, 1e18 away along +x (targets) or -x (sources), each at its
+// own distance.  Every pair with a pad then has r^3 m = 0 exactly and all
+// intermediates finite (s ~ 1e36 fits FP32), so pads add nothing to real
+// bodies; their own sums are never written.
+__device__ __forceinline__ double pad_x(int k, bool source) { return (source ? -1e18 : 1e18) * (1.0 + 1e-3 * (k & 1023)); }
 
 // Row rotation by R lanes: lane k of each 16-lane row receives the value of
 // lane (k - R) mod 16 of the same row.
@@ -84,15 +93,26 @@ struct SymMixed {
     struct Tot {
         float hi[K][6], lo[K][6];
     };
-    static __device__ __forceinline__ Rec load_rec(const State& s, int j) { return load_jbmx(s, j); }
-    static __device__ __forceinline__ void load_targets(const State& s, int ibase, Targets& I)
+    static __device__ __forceinline__ Rec load_rec(const State& s, int j, int n)
+    {
+        if (j < n) return load_jbmx(s, j);
+        Rec b{};
+        b.x = pad_x(j, true);
+        return b;
+    }
+    static __device__ __forceinline__ void load_targets(const State& s, int ibase, int n, Targets& I)
     {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = ibase + k * 64;
-            I.x[k] = s.x[i]; I.y[k] = s.y[i]; I.z[k] = s.z[i];
-            I.m[k] = (float)s.m[i];
-            I.vx[k] = (float)s.vx[i]; I.vy[k] = (float)s.vy[i]; I.vz[k] = (float)s.vz[i];
+            const int ic = i < n ? i : n - 1;
+            I.x[k] = s.x[ic]; I.y[k] = s.y[ic]; I.z[k] = s.z[ic];
+            I.m[k] = (float)s.m[ic];
+            I.vx[k] = (float)s.vx[ic]; I.vy[k] = (float)s.vy[ic]; I.vz[k] = (float)s.vz[ic];
+            if (i >= n) {
+                I.x[k] = pad_x(i, false); I.y[k] = I.z[k] = 0.0;
+                I.m[k] = I.vx[k] = I.vy[k] = I.vz[k] = 0.f;
+            }
         }
     }
     template <int R>
@@ -171,15 +191,26 @@ struct SymF64 {
     struct Tot {
         double t[K][6];
     };
-    static __device__ __forceinline__ Rec load_rec(const State& s, int j) { return load_jb64(s, j); }
-    static __device__ __forceinline__ void load_targets(const State& s, int ibase, Targets& I)
+    static __device__ __forceinline__ Rec load_rec(const State& s, int j, int n)
+    {
+        if (j < n) return load_jb64(s, j);
+        Rec b{};
+        b.x = pad_x(j, true);
+        return b;
+    }
+    static __device__ __forceinline__ void load_targets(const State& s, int ibase, int n, Targets& I)
     {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = ibase + k * 64;
-            I.x[k] = s.x[i]; I.y[k] = s.y[i]; I.z[k] = s.z[i];
-            I.m[k] = s.m[i];
-            I.vx[k] = s.vx[i]; I.vy[k] = s.vy[i]; I.vz[k] = s.vz[i];
+            const int ic = i < n ? i : n - 1;
+            I.x[k] = s.x[ic]; I.y[k] = s.y[ic]; I.z[k] = s.z[ic];
+            I.m[k] = s.m[ic];
+            I.vx[k] = s.vx[ic]; I.vy[k] = s.vy[ic]; I.vz[k] = s.vz[ic];
+            if (i >= n) {
+                I.x[k] = pad_x(i, false); I.y[k] = I.z[k] = 0.0;
+                I.m[k] = I.vx[k] = I.vy[k] = I.vz[k] = 0.0;
+            }
         }
     }
     template <int R>
@@ -251,7 +282,8 @@ __device__ __forceinline__ void sym_group(const typename T::Rec* __restrict__ ro
 
 struct SymArgs {
     State s;
-    int nb;       // blocks of 64*K*W bodies (n = nb * block)
+    int n;        // bodies
+    int nb;       // blocks of 64*K*W bodies, the last one possibly partial
     int half;     // H = (nb - 1) / 2
     int chunk;    // block pairs per workgroup
     double eps2;
@@ -288,7 +320,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
     const int ibase = P * B + w * 64 * K + lane;
 
     typename T::Targets I;
-    T::load_targets(a.s, ibase, I);
+    T::load_targets(a.s, ibase, a.n, I);
     // first source of tile tt of this workgroup's range
     auto tile_src = [&](int tt) {
         int q = P + d0 + tt / TPB;
@@ -298,7 +330,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
 
     typename T::Rec st{};
     if (t < kSymTile) {
-        st = T::load_rec(a.s, tile_src(0) + t);
+        st = T::load_rec(a.s, tile_src(0) + t, a.n);
         dup[0][t >> 4][t & 15] = st;
         dup[0][t >> 4][(t & 15) + 16] = st;
     }
@@ -333,7 +365,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
         // the next tile is loaded after the arithmetic, not held in registers
         // across it (12 VGPRs); the other resident workgroups cover the load
         if (more && t < kSymTile) {
-            st = T::load_rec(a.s, tile_src(tt + 1) + t);
+            st = T::load_rec(a.s, tile_src(tt + 1) + t, a.n);
             dup[buf ^ 1][t >> 4][t & 15] = st;
             dup[buf ^ 1][t >> 4][(t & 15) + 16] = st;
         }
@@ -343,16 +375,19 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
         for (int e = t; e < kSymTile * 6; e += 64 * W) {
             const int c = e / kSymTile;
             const int j = e % kSymTile;
-            double sum = 0.0;
+            if (j0 + j < a.n) {
+                double sum = 0.0;
 #pragma unroll
-            for (int v = 0; v < W; ++v) sum += (double)part[buf][v][j][c];
-            unsafeAtomicAdd(&a.out.c[c][j0 + j], sum);
+                for (int v = 0; v < W; ++v) sum += (double)part[buf][v][j][c];
+                unsafeAtomicAdd(&a.out.c[c][j0 + j], sum);
+            }
         }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) unsafeAtomicAdd(&a.out.c[c][ibase + k * 64], T::total(tot, k, c));
+        for (int c = 0; c < 6; ++c)
+            if (ibase + k * 64 < a.n) unsafeAtomicAdd(&a.out.c[c][ibase + k * 64], T::total(tot, k, c));
 }
 
 // Launch shape per precision: targets per lane, waves per workgroup.
@@ -363,15 +398,12 @@ constexpr int kF64K = 2, kF64W = 4;
 
 int sym_block(int precision) { return precision == 0 ? 64 * kF64K * kF64W : 64 * kMixK * kMixW; }
 
-bool sym_supported(int precision, int n)
-{
-    const int b = sym_block(precision);
-    return n % b == 0 && n >= 2 * b;
-}
+bool sym_supported(int precision, int n) { return n >= 2 * sym_block(precision); }
 
 int sym_chunk(int precision, int n, int cus)
 {
-    const int nb = n / sym_block(precision);
+    const int b = sym_block(precision);
+    const int nb = (n + b - 1) / b;
     const int pairs_per_block = (nb - 1) / 2 + 1;
     // about 64 workgroups per CU in all (some 16 rounds of resident ones),
     // so the last round is a small part; measured at n = 2^18 and 2^20
@@ -393,7 +425,8 @@ hipError_t launch_force_sym(int precision, const State& s, int n, double eps2, i
     if ((e = launch_force_blockdiag(precision, s, n, eps2, B, out, st)) != hipSuccess) return e;
     SymArgs a;
     a.s = s;
-    a.nb = n / B;
+    a.n = n;
+    a.nb = (n + B - 1) / B;
     a.half = (a.nb - 1) / 2;
     a.chunk = chunk;
     a.eps2 = eps2;

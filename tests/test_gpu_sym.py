@@ -25,9 +25,10 @@ def force(nb, state, eps2, pairing, precision="mixed"):
         return b.force(state), shape
 
 
-# n = 1024 .. 2560: 2, 3, 4, 5 blocks of 512 (even and odd ring)
+# n = 1024 .. 2560: 2, 3, 4, 5 blocks of 512 (even and odd ring); 1025, 1100,
+# 2500, 3001: a partial last block (1, 76, 452, 441 bodies) padded inside
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-@pytest.mark.parametrize("n", [1024, 1536, 2048, 2560])
+@pytest.mark.parametrize("n", [1024, 1536, 2048, 2560, 1025, 1100, 2500, 3001])
 def test_symmetric_matches_oracle(nb, oracle, n, precision):
     _banner(f"oracle {precision} n={n}")
     s = nb.plummer(n, 11 + n)
@@ -67,7 +68,7 @@ def test_symmetric_close_to_directed(nb, precision):
 def test_third_law_momentum(nb, precision, rel):
     """sum_i m_i a_i = 0 and sum_i m_i jerk_i = 0: each pair adds equal and
     opposite terms, so the totals vanish to the rounding of the sums."""
-    n = 4096
+    n = 4000
     s = nb.plummer(n, 7)
     s[3] *= np.exp(0.3 * np.random.default_rng(3).standard_normal(n))
     f, _ = force(nb, s, 1e-5, "symmetric", precision)
@@ -79,9 +80,10 @@ def test_third_law_momentum(nb, precision, rel):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_eps2_zero_diagonal_pass(nb, oracle, precision):
-    """eps2 = 0: the self pair is dropped in the block-diagonal pass."""
-    n = 2048
+@pytest.mark.parametrize("n", [2048, 2100])
+def test_eps2_zero_diagonal_pass(nb, oracle, precision, n):
+    """eps2 = 0: the self pair is dropped in the block-diagonal pass, and the
+    pad bodies of a partial block stay away from every body."""
     s = nb.plummer(n, 13)
     f, _ = force(nb, s, 0.0, "symmetric", precision)
     assert np.isfinite(f).all()
@@ -91,7 +93,7 @@ def test_eps2_zero_diagonal_pass(nb, oracle, precision):
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_unsupported_n_runs_directed(nb, precision):
-    for n in (1000, 512, 1023, 3000):
+    for n in (1000, 512, 1023, 7):
         s = nb.plummer(n, 2)
         fs, shape = force(nb, s, 1e-4, "symmetric", precision)
         assert shape["pairing"] == "directed"
@@ -131,10 +133,10 @@ def test_large_sample_vs_oracle(nb, oracle, precision):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_resident_steps_and_graph(nb, precision):
+@pytest.mark.parametrize("n", [4096, 4100])
+def test_resident_steps_and_graph(nb, precision, n):
     """The integrator on the symmetric kernel: eager and graph replay agree to
     the atomics' rounding, and the energy drift matches the directed kernel's."""
-    n = 4096
     s = nb.plummer(n, 21)
     eps2 = (1 / 64) ** 2
     res = {}
