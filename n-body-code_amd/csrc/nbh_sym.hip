@@ -177,12 +177,20 @@ struct SymMixed {
     }
 };
 
+// FP64 source record of the symmetric kernel: 80 bytes, so that the 16
+// lanes of a row, reading 16 consecutive records, start on 16 different LDS
+// banks (20 dwords apart; a 64-byte record would put lanes p and p + 4 on
+// the same bank).  pad is never read.
+struct __align__(16) JB80 {
+    double x, y, z, m, vx, vy, vz, pad[3];
+};
+
 // FP64: the directed FP64 kernel's arithmetic (pair_f64) with the reaction
 // added; per-tile sums added to FP64 totals.
 template <int K>
 struct SymF64 {
     static constexpr int kWaves = 3;
-    using Rec = JB64;
+    using Rec = JB80;
     using Sum = double;
     using Eps = double;
     struct Targets {
@@ -193,9 +201,13 @@ struct SymF64 {
     };
     static __device__ __forceinline__ Rec load_rec(const State& s, int j, int n)
     {
-        if (j < n) return load_jb64(s, j);
         Rec b{};
-        b.x = pad_x(j, true);
+        if (j < n) {
+            b.x = s.x[j]; b.y = s.y[j]; b.z = s.z[j]; b.m = s.m[j];
+            b.vx = s.vx[j]; b.vy = s.vy[j]; b.vz = s.vz[j];
+        } else {
+            b.x = pad_x(j, true);
+        }
         return b;
     }
     static __device__ __forceinline__ void load_targets(const State& s, int ibase, int n, Targets& I)
