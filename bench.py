@@ -185,7 +185,9 @@ def roofline(prec, res, traffic):
 def run_ensemble(nb, log, nsteps=64):
     """8192 independent Plummer clusters, n_k log-uniform in [8, 4096]
     (seed 4), eps_k = 1/(4 n_k^(1/3)), dt = 1/256: autotuned persistent
-    force launch, steps replayed from a hipGraph."""
+    force launch, steps replayed from a hipGraph.  The *_symmetric legs
+    evaluate systems of >= 1024 bodies with symmetric pairing (a second
+    launch beside the persistent one)."""
     counts = nb.ensemble_sizes(8192, 8, 4096, 4)
     eps2 = (1.0 / (4.0 * counts.astype(float) ** (1.0 / 3.0))) ** 2
     state = nb.ensemble_state(counts, 4)
@@ -193,8 +195,10 @@ def run_ensemble(nb, log, nsteps=64):
     out = {"workload": "8192 Plummer clusters, n_k log-uniform in [8,4096] (seed 4), "
                        "eps_k=1/(4 n_k^(1/3)), dt=1/256, 64 steps, hipGraph replay",
            "n_systems": int(len(counts)), "n_bodies": int(counts.sum()), "pairs_per_step": pairs}
-    for prec in ("mixed", "fp64"):
-        with nb.Backend(prec) as b, nb.Ensemble(b, counts, eps2) as e:
+    for leg in ("mixed", "fp64", "mixed_symmetric", "fp64_symmetric"):
+        prec = leg.split("_")[0]
+        pairing = "symmetric" if leg.endswith("symmetric") else "directed"
+        with nb.Backend(prec, pairing=pairing) as b, nb.Ensemble(b, counts, eps2) as e:
             e.upload(state)
             tune = e.autotune()
             e.set_dt(1.0 / 256)
@@ -206,11 +210,11 @@ def run_ensemble(nb, log, nsteps=64):
             k1, u1 = e.energy()
             info = e.info()
         drift = np.abs((k1 + u1) - (k0 + u0)) / np.abs(k0 + u0)
-        out[prec] = {"systems_per_s": len(counts) * nsteps / sec, "pairs_per_s": pairs * nsteps / sec,
+        out[leg] = {"systems_per_s": len(counts) * nsteps / sec, "pairs_per_s": pairs * nsteps / sec,
                      "steps_per_s": nsteps / sec, "ms_per_step": 1e3 * sec / nsteps,
                      "energy_drift_median": float(np.median(drift)), "energy_drift_max": float(drift.max()),
                      "tuning": tune, "work_items": info["items"], "workgroups": info["workgroups"]}
-        log(f"ensemble {prec}: {1e3 * sec / nsteps:.2f} ms/step, {pairs * nsteps / sec:.3e} pairs/s, "
+        log(f"ensemble {leg}: {1e3 * sec / nsteps:.2f} ms/step, {pairs * nsteps / sec:.3e} pairs/s, "
             f"{len(counts) * nsteps / sec:.3e} systems/s, drift median {np.median(drift):.1e}, tuning {tune}")
     return out
 

@@ -200,8 +200,12 @@ int nbh_device_info(int device, int *compute_units, int *wave_size,
  * largest first; a size bucket's work items use the LDS tile length and
  * j-split chosen for that bucket, and all items one or two targets per lane
  * (by nbh_ens_autotune or nbh_ens_set_tuning / nbh_ens_set_targets_per_lane).
- * Precision and device come from the context.  The step is the shared-dt
- * Hermite step of nbh_step applied to every system. */
+ * Precision and device come from the context.  When the context was created
+ * with NBH_PAIR_SYMMETRIC, systems of >= 1024 bodies are evaluated with
+ * symmetric pairing instead: the persistent launch covers only their
+ * diagonal 512-body blocks, and a second launch adds every block pair once
+ * (FP64 atomics, as in nbh_force).  The step is the shared-dt Hermite step
+ * of nbh_step applied to every system. */
 
 typedef struct nbh_ensemble nbh_ensemble;
 

@@ -325,9 +325,11 @@ struct nbh_ensemble {
     int split[NBH_ENS_BUCKETS] = {1, 1, 1, 1};
     int ipl = 1;  // targets per lane, every work item
 
+    bool sym = false;  // systems of >= 2 symmetric blocks use symmetric pairing
+
     // device
-    DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot;
-    int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0;
+    DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot, symitems;
+    int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0, nsym = 0;
     int list_ipl = 1;  // targets per lane of the uploaded work list
     bool uploaded = false, dt_set = false;
     hipGraphExec_t gexec = nullptr;
@@ -347,14 +349,21 @@ void drop_graph(nbh_ensemble* e)
 }
 
 // Work list for the systems selected by `use` (all when empty), with the
-// per-bucket tile/split of `tile`/`split` and `ipl` targets per lane.
+// per-bucket tile/split of `tile`/`split` and `ipl` targets per lane.  With
+// symmetric pairing, a system of at least two symmetric blocks gets directed
+// items for its diagonal blocks only (each target against its own block,
+// written directly) and symmetric items for its block pairs, which the
+// symmetric kernel adds afterwards.
 void build_items(const nbh_ensemble* e, const int* tile, const int* split, int ipl,
                  const std::vector<char>& use, std::vector<EnsItem>& items,
-                 std::vector<EnsReduce>& reds, int& nslots)
+                 std::vector<EnsReduce>& reds, int& nslots, std::vector<nbh::SymItem>& sym)
 {
     items.clear();
     reds.clear();
+    sym.clear();
     nslots = 0;
+    const int prec = e->ctx->cfg.precision;
+    const int SB = nbh::sym_block(prec);
     for (int k = 0; k < e->nsys; ++k) {
         if (!use.empty() && !use[k]) continue;
         const int n = e->cnt[k], off = e->off[k];
@@ -362,6 +371,18 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
         const int T = tile[b];
         const int Q = ipl;
         const int per = nbh::kLanes * Q;
+        if (e->sym && nbh::sym_supported(prec, n)) {
+            const int nb = (n + SB - 1) / SB;
+            const int half = (nb - 1) / 2;
+            for (int P = 0; P < nb; ++P) {
+                const int b0 = P * SB, b1 = std::min(n, b0 + SB);
+                for (int ib = b0; ib < b1; ib += per)
+                    items.push_back({k, off + ib, std::min(per, b1 - ib), off + b0, off + b1, T, -1, Q});
+                const int dmax = half + ((nb % 2 == 0 && P < nb / 2) ? 1 : 0);
+                if (dmax >= 1) sym.push_back({off, n, nb, P, 1, dmax, e->eps2[k]});
+            }
+            continue;
+        }
         int S = std::min(split[b], (n + T - 1) / T);
         S = std::max(S, 1);
         const int chunk = (n + S - 1) / S;
@@ -379,16 +400,21 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
             }
         }
     }
+    // most block pairs first
+    std::stable_sort(sym.begin(), sym.end(), [](const nbh::SymItem& a, const nbh::SymItem& b) {
+        return (long)(a.d1 - a.d0 + 1) * a.n > (long)(b.d1 - b.d0 + 1) * b.n;
+    });
     // largest source range first; stable keeps system order among equals
     std::stable_sort(items.begin(), items.end(),
                      [](const EnsItem& a, const EnsItem& b) { return (a.j1 - a.j0) > (b.j1 - b.j0); });
 }
 
 int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::vector<EnsReduce>& reds,
-                 int nslots, int maxtile, int ipl)
+                 int nslots, int maxtile, int ipl, const std::vector<nbh::SymItem>& sym)
 {
     nbh_ctx* c = e->ctx;
-    if (e->items.reserve(sizeof(EnsItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
+    if (e->symitems.reserve(sizeof(nbh::SymItem) * std::max<size_t>(sym.size(), 1)) != hipSuccess ||
+        e->items.reserve(sizeof(EnsItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
         e->reduces.reserve(sizeof(EnsReduce) * std::max<size_t>(reds.size(), 1)) != hipSuccess ||
         e->partial.reserve(sizeof(double) * 6 * nbh::kMaxTargets * std::max(nslots, 1)) != hipSuccess)
         return fail(NBH_ERR_ALLOC, "ensemble work-list allocation failed");
@@ -398,9 +424,13 @@ int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::
     if (!reds.empty())
         NBH_HIP(hipMemcpyAsync(e->reduces.p, reds.data(), sizeof(EnsReduce) * reds.size(),
                                hipMemcpyHostToDevice, c->stream));
+    if (!sym.empty())
+        NBH_HIP(hipMemcpyAsync(e->symitems.p, sym.data(), sizeof(nbh::SymItem) * sym.size(),
+                               hipMemcpyHostToDevice, c->stream));
     NBH_HIP(hipStreamSynchronize(c->stream));
     e->nitems = (int)items.size();
     e->nreduce = (int)reds.size();
+    e->nsym = (int)sym.size();
     e->nslots = nslots;
     e->maxtile = maxtile;
     e->list_ipl = ipl;
@@ -416,12 +446,13 @@ int rebuild(nbh_ensemble* e)
 {
     std::vector<EnsItem> items;
     std::vector<EnsReduce> reds;
+    std::vector<nbh::SymItem> sym;
     int nslots = 0;
-    build_items(e, e->tile, e->split, e->ipl, {}, items, reds, nslots);
+    build_items(e, e->tile, e->split, e->ipl, {}, items, reds, nslots, sym);
     int maxt = 64;
     for (const auto& it : items) maxt = std::max(maxt, it.tile);
     drop_graph(e);
-    return upload_items(e, items, reds, nslots, maxt, e->ipl);
+    return upload_items(e, items, reds, nslots, maxt, e->ipl, sym);
 }
 
 // Force evaluation of the current work list: state -> out.
@@ -449,7 +480,7 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
     if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d", e->maxtile);
     void* args[] = {&a};
     NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, c->stream));
-    if (timed) {
+    if (timed && e->nsym == 0) {
         NBH_HIP(hipEventRecord(ep.b, c->stream));
         c->ev_pending.push_back(ep);
     }
@@ -457,6 +488,14 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
         nbh::k_ens_reduce<<<e->nreduce, nbh::kLanes, 0, c->stream>>>(static_cast<const EnsReduce*>(e->reduces.p),
                                                                       e->nreduce, e->partial.d(), out);
         NBH_HIP(hipGetLastError());
+    }
+    if (e->nsym > 0) {
+        NBH_HIP(nbh::launch_ens_sym(c->cfg.precision, a.s, out, static_cast<const nbh::SymItem*>(e->symitems.p),
+                                    e->nsym, c->stream));
+        if (timed) {
+            NBH_HIP(hipEventRecord(ep.b, c->stream));
+            c->ev_pending.push_back(ep);
+        }
     }
     return NBH_OK;
 }
@@ -500,6 +539,7 @@ int nbh_ens_create(nbh_ctx* ctx, int nsys, const int* counts, const double* eps2
     auto* e = new (std::nothrow) nbh_ensemble();
     if (!e) return fail(NBH_ERR_ALLOC, "ensemble allocation failed");
     e->ctx = ctx;
+    e->sym = ctx->cfg.pairing == NBH_PAIR_SYMMETRIC;
     e->nsys = nsys;
     e->total = (int)total;
     e->cnt.assign(counts, counts + nsys);
@@ -542,7 +582,7 @@ int nbh_ens_destroy(nbh_ensemble* e)
     (void)hipStreamSynchronize(e->ctx->stream);
     drop_graph(e);
     for (DevBuf* b : {&e->sys, &e->state, &e->f0, &e->f1, &e->items, &e->reduces, &e->partial, &e->counter,
-                      &e->dt, &e->ekin, &e->epot})
+                      &e->dt, &e->ekin, &e->epot, &e->symitems})
         b->release();
     delete e;
     return NBH_OK;
@@ -724,10 +764,11 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
                 sp[bk] = S;
                 std::vector<EnsItem> items;
                 std::vector<EnsReduce> reds;
+                std::vector<nbh::SymItem> sym;
                 int nslots = 0;
-                build_items(e, tl, sp, e->ipl, use, items, reds, nslots);
+                build_items(e, tl, sp, e->ipl, use, items, reds, nslots, sym);
                 double ms = 0.0;
-                if ((rc = upload_items(e, items, reds, nslots, T, e->ipl)) || (rc = time_list(&ms))) break;
+                if ((rc = upload_items(e, items, reds, nslots, T, e->ipl, sym)) || (rc = time_list(&ms))) break;
                 if (ms < best) {
                     best = ms;
                     bt = T;

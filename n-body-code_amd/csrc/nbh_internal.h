@@ -61,6 +61,17 @@ hipError_t launch_force_sym(int precision, const State& s, int n, double eps2,
                             int chunk, const Forces& out, hipStream_t st,
                             hipEvent_t ev0, hipEvent_t ev1);
 
+// One work item of the list-driven symmetric kernel (the ensemble): block P
+// of the system occupying [off, off + n) against blocks P + d0 .. P + d1
+// (mod nb, d0 >= 1) of the same system, with its eps2.  The outputs must
+// already hold the block-diagonal sums: the items add to them atomically.
+struct SymItem {
+    int off, n, nb, P, d0, d1;
+    double eps2;
+};
+hipError_t launch_ens_sym(int precision, const State& s, const Forces& out,
+                          const SymItem* items, int nitems, hipStream_t st);
+
 // Hermite predictor in place (x, v) from forces f0 with dt read from *dt.
 hipError_t launch_predict(int n, const double* dt, double* x, double* y,
                           double* z, double* vx, double* vy, double* vz,

@@ -302,13 +302,11 @@ struct SymArgs {
     Forces out;   // initialised by the block-diagonal pass
 };
 
-// Resident waves per SIMD asked of the compiler: T::kWaves (mixed 5: 96
-// VGPRs, a few spills; FP64 3).  gfx950 issues FP32 at twice the one-wave
-// rate only by pairing instructions of two waves, so more resident waves
-// raise the dual-issue share (mixed: +1.6% at n = 2^20 over 4 waves).
+// Block P of one system (bodies 0..n-1 of s and out) against blocks
+// P + d0 .. P + d1 (mod nb), d0 >= 1: the body of both symmetric kernels.
 template <template <int> class TT, int K, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::kWaves, 8))) void k_force_sym(
-    SymArgs a)
+__device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& out, int n, int nb, double eps2d,
+                                                int P, int d0, int d1)
 {
     using T = TT<K>;
     using Sum = typename T::Sum;
@@ -316,33 +314,27 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
     constexpr int TPB = B / kSymTile;  // tiles per block
     __shared__ typename T::Rec dup[2][4][32];
     __shared__ Sum part[2][W][kSymTile][6];
-
-    const int P = blockIdx.x;
-    const int dmax = a.half + ((a.nb % 2 == 0 && P < a.nb / 2) ? 1 : 0);
-    const int d0 = 1 + blockIdx.y * a.chunk;
-    if (d0 > dmax) return;  // uniform over the workgroup, before any barrier
-    const int d1 = min(dmax, d0 + a.chunk - 1);
     const int ntiles = (d1 - d0 + 1) * TPB;
 
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
     const int p = lane & 15;
-    const typename T::Eps eps2 = (typename T::Eps)a.eps2;
+    const typename T::Eps eps2 = (typename T::Eps)eps2d;
     const int ibase = P * B + w * 64 * K + lane;
 
     typename T::Targets I;
-    T::load_targets(a.s, ibase, a.n, I);
+    T::load_targets(s, ibase, n, I);
     // first source of tile tt of this workgroup's range
     auto tile_src = [&](int tt) {
         int q = P + d0 + tt / TPB;
-        if (q >= a.nb) q -= a.nb;
+        if (q >= nb) q -= nb;
         return q * B + (tt % TPB) * kSymTile;
     };
 
     typename T::Rec st{};
     if (t < kSymTile) {
-        st = T::load_rec(a.s, tile_src(0) + t, a.n);
+        st = T::load_rec(s, tile_src(0) + t, n);
         dup[0][t >> 4][t & 15] = st;
         dup[0][t >> 4][(t & 15) + 16] = st;
     }
@@ -377,7 +369,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
         // the next tile is loaded after the arithmetic, not held in registers
         // across it (12 VGPRs); the other resident workgroups cover the load
         if (more && t < kSymTile) {
-            st = T::load_rec(a.s, tile_src(tt + 1) + t, a.n);
+            st = T::load_rec(s, tile_src(tt + 1) + t, n);
             dup[buf ^ 1][t >> 4][t & 15] = st;
             dup[buf ^ 1][t >> 4][(t & 15) + 16] = st;
         }
@@ -387,11 +379,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
         for (int e = t; e < kSymTile * 6; e += 64 * W) {
             const int c = e / kSymTile;
             const int j = e % kSymTile;
-            if (j0 + j < a.n) {
+            if (j0 + j < n) {
                 double sum = 0.0;
 #pragma unroll
                 for (int v = 0; v < W; ++v) sum += (double)part[buf][v][j][c];
-                unsafeAtomicAdd(&a.out.c[c][j0 + j], sum);
+                unsafeAtomicAdd(&out.c[c][j0 + j], sum);
             }
         }
     }
@@ -399,7 +391,39 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
     for (int k = 0; k < K; ++k)
 #pragma unroll
         for (int c = 0; c < 6; ++c)
-            if (ibase + k * 64 < a.n) unsafeAtomicAdd(&a.out.c[c][ibase + k * 64], T::total(tot, k, c));
+            if (ibase + k * 64 < n) unsafeAtomicAdd(&out.c[c][ibase + k * 64], T::total(tot, k, c));
+}
+
+// Resident waves per SIMD asked of the compiler: T::kWaves (mixed 5: 96
+// VGPRs, a few spills; FP64 3).  gfx950 issues FP32 at twice the one-wave
+// rate only by pairing instructions of two waves, so more resident waves
+// raise the dual-issue share (mixed: +1.6% at n = 2^20 over 4 waves).
+template <template <int> class TT, int K, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::kWaves, 8))) void k_force_sym(
+    SymArgs a)
+{
+    const int P = blockIdx.x;
+    const int dmax = a.half + ((a.nb % 2 == 0 && P < a.nb / 2) ? 1 : 0);
+    const int d0 = 1 + blockIdx.y * a.chunk;
+    if (d0 > dmax) return;  // uniform over the workgroup, before any barrier
+    const int d1 = min(dmax, d0 + a.chunk - 1);
+    sym_block_pairs<TT, K, W>(a.s, a.out, a.n, a.nb, a.eps2, P, d0, d1);
+}
+
+// Many systems: one workgroup per item of a list (system offset and size,
+// block P and its block-pair range), systems' arrays concatenated.
+template <template <int> class TT, int K, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::kWaves, 8))) void k_ens_sym(
+    State s, Forces out, const SymItem* __restrict__ items)
+{
+    const SymItem it = items[blockIdx.x];
+    State q = s;
+    q.x += it.off; q.y += it.off; q.z += it.off; q.m += it.off;
+    q.vx += it.off; q.vy += it.off; q.vz += it.off;
+    Forces o = out;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o.c[c] += it.off;
+    sym_block_pairs<TT, K, W>(q, o, it.n, it.nb, it.eps2, it.P, it.d0, it.d1);
 }
 
 // Launch shape per precision: targets per lane, waves per workgroup.
@@ -425,6 +449,17 @@ int sym_chunk(int precision, int n, int cus)
     if (c < 1) c = 1;
     if (c > pairs_per_block) c = pairs_per_block;
     return (int)c;
+}
+
+hipError_t launch_ens_sym(int precision, const State& s, const Forces& out, const SymItem* items, int nitems,
+                          hipStream_t st)
+{
+    if (nitems <= 0) return hipSuccess;
+    if (precision == 0)
+        k_ens_sym<SymF64, kF64K, kF64W><<<nitems, 64 * kF64W, 0, st>>>(s, out, items);
+    else
+        k_ens_sym<SymMixed, kMixK, kMixW><<<nitems, 64 * kMixW, 0, st>>>(s, out, items);
+    return hipGetLastError();
 }
 
 hipError_t launch_force_sym(int precision, const State& s, int n, double eps2, int chunk, const Forces& out,

@@ -198,3 +198,45 @@ def test_bad_arguments(nb):
             with pytest.raises(nb.NBodyError) as ex:
                 e.set_targets_per_lane(3)
             assert ex.value.code == nb.ERR_ARG
+
+
+# ---- symmetric pairing (context created with pairing="symmetric"): systems
+# of >= 1024 bodies evaluate each unordered pair once
+SYM_SIZES = [1024, 1025, 1500, 2048, 3001, 600, 100, 5, 1]
+
+
+@pytest.fixture(scope="module")
+def symset(nb):
+    counts = np.array(SYM_SIZES, dtype=np.int32)
+    eps2 = (1.0 / (4.0 * counts.astype(float) ** (1 / 3))) ** 2
+    state = nb.ensemble_state(counts, 23)
+    return counts, eps2, state
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_symmetric_ensemble_vs_oracle(nb, oracle, symset, precision):
+    counts, eps2, state = symset
+    with nb.Backend(precision, pairing="symmetric") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.upload(state)
+        _, f = e.download()
+        w = check_systems(e, f, state, oracle, eps2, mixed=(precision == "mixed"))
+    _banner(f"symmetric {precision} ensemble {SYM_SIZES}: worst fp64 per-body rel err {w:.2e}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_symmetric_ensemble_matches_directed_and_graph(nb, symset, precision):
+    counts, eps2, state = symset
+    out = {}
+    for pairing, graph in (("directed", False), ("symmetric", False), ("symmetric", True)):
+        with nb.Backend(precision, pairing=pairing) as b, nb.Ensemble(b, counts, eps2) as e:
+            e.upload(state)
+            e.autotune()
+            e.set_dt(1 / 256)
+            e.step(16, graph=graph)
+            out[(pairing, graph)] = e.download()
+    xd = out[("directed", False)][0]
+    xs, xg = out[("symmetric", False)][0], out[("symmetric", True)][0]
+    d = np.abs(xs[:3] - xd[:3]).max()
+    _banner(f"{precision}: 16 steps, max |x_sym - x_dir| {d:.2e}, |graph - eager| {np.abs(xs - xg).max():.2e}")
+    assert d <= (1e-12 if precision == "fp64" else 1e-6)
+    assert np.abs(xs - xg).max() <= (1e-13 if precision == "fp64" else 1e-9)
