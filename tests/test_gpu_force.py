@@ -217,3 +217,35 @@ def test_force_device_pointers(be64, oracle, nb):
         hip.hipFree(d_in)
         hip.hipFree(d_out)
     check64(out, oracle.force(s, 1e-4))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_force_device_pointers_symmetric(oracle, nb, precision):
+    """nbh_force_device with symmetric pairing: the atomic totals land in the
+    caller's hipMalloc'd outputs (n = 2100: a partial last block)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    n = 2100
+    s = np.ascontiguousarray(nb.plummer(n, 22))
+    out = np.zeros((6, n))
+    d_in, d_out = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(d_in), C.c_size_t(s.nbytes)) == 0
+    assert hip.hipMalloc(C.byref(d_out), C.c_size_t(out.nbytes)) == 0
+    try:
+        assert hip.hipMemcpy(d_in, s.ctypes.data_as(C.c_void_p), C.c_size_t(s.nbytes), 1) == 0
+        with nb.Backend(precision, 1e-4, pairing="symmetric") as b:
+            assert b.launch_shape(n)["pairing"] == "symmetric"
+            col = lambda base, k: C.c_void_p(base.value + 8 * n * k)  # noqa: E731
+            rc = nb.lib().nbh_force_device(b._h, n, *[col(d_in, k) for k in range(7)],
+                                           *[col(d_out, k) for k in range(6)], None)
+            assert rc == nb.OK
+            b.timing()  # drains the context stream
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), d_out, C.c_size_t(out.nbytes), 2) == 0
+    finally:
+        hip.hipFree(d_in)
+        hip.hipFree(d_out)
+    ref = oracle.force(s, 1e-4)
+    if precision == "fp64":
+        check64(out, ref)
+    else:
+        assert np.median(body_relerr(out[:3], ref[:3])) <= 1e-6
