@@ -251,26 +251,9 @@ def run_adjoint(nb, log):
     return out
 
 
-def cpu_baseline(state, budget_s=12.0):
-    """The reference's CPU FP64 force_calculation (src/kernel.cpp:1164-1224,
-    compiled from its own sources into oracle/_ref, EPS2 = 1e-7: same work per
-    pair) on the first M bodies of the same sphere, with OpenMP threads on
-    this host's cores; when that build is absent, the oracle's restatement of
-    the same loop.  M is sized for ~budget_s of CPU time."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle, Reference
-    cores = max(1, min(16, os.cpu_count() or 1))
-    try:
-        R = Reference("fp64")
-        kind = "reference"
-        run = lambda s: R.force(s, nthreads=cores)
-        what = "the reference's own FP64 force_calculation (compiled from its sources, EPS2=1e-7)"
-    except (RuntimeError, OSError):
-        O = Oracle()
-        kind = "port"
-        run = lambda s: O.force(s, EPS * EPS, "fp64", nthreads=cores)
-        what = "the oracle's restatement of the reference FP64 loop (eps=1e-3)"
-    m0 = 8192
+def _time_force(run, state, budget_s):
+    """One evaluation of run() on the first M bodies, M sized for ~budget_s."""
+    m0 = 16384
     t = time.perf_counter()
     run(state[:, :m0])
     rate = m0 * (m0 - 1) / (time.perf_counter() - t)
@@ -279,11 +262,44 @@ def cpu_baseline(state, budget_s=12.0):
     t = time.perf_counter()
     run(state[:, :m])
     sec = time.perf_counter() - t
-    return {"value": m * (m - 1) / sec, "unit": "pair-interactions/s", "cores": cores,
-            "kind": kind,
-            "sample": f"one evaluation of {what}, triangle order, over the first {m} bodies of the "
-                      f"N=1048576 seed-3 Plummer sphere, {cores} OpenMP threads, {sec:.1f} s; "
-                      f"rate = M(M-1)/t directed pairs"}
+    return m, sec, m * (m - 1) / sec
+
+
+def cpu_baseline(state, budget_s=12.0):
+    """The reference's CPU force_calculation, FP64 (src/kernel.cpp:1164-1224)
+    and _F32 (src/kernel.cpp:1101-1163) builds, compiled from its own sources
+    into oracle/_ref (EPS2 = 1e-7: same work per pair), on the first M bodies
+    of the same sphere with OpenMP threads on this host's cores; when that
+    build is absent, the oracle's restatement of the same loops.  M is sized
+    for ~budget_s of CPU time each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, Reference
+    cores = max(1, min(16, os.cpu_count() or 1))
+    runs = {}
+    try:
+        for prec in ("fp64", "f32"):
+            R = Reference(prec)
+            runs[prec] = (lambda s, R=R: R.force(s, nthreads=cores))
+        kind = "reference"
+        what = "the reference's own {} force_calculation (compiled from its sources, EPS2=1e-7)"
+    except (RuntimeError, OSError):
+        O = Oracle()
+        for prec in ("fp64", "f32"):
+            runs[prec] = (lambda s, p=prec: O.force(s, EPS * EPS, p, nthreads=cores))
+        kind = "port"
+        what = "the oracle's restatement of the reference {} loop (eps=1e-3)"
+    out = {}
+    for prec in ("fp64", "f32"):
+        m, sec, rate = _time_force(runs[prec], state, budget_s)
+        out[prec] = (m, sec, rate)
+    m, sec, rate = out["fp64"]
+    m32, sec32, rate32 = out["f32"]
+    return {"value": rate, "unit": "pair-interactions/s", "cores": cores, "kind": kind,
+            "sample": f"one evaluation of {what.format('FP64')}, triangle order, over the first {m} bodies of "
+                      f"the N=1048576 seed-3 Plummer sphere, {cores} OpenMP threads (one process), {sec:.1f} s; "
+                      f"rate = M(M-1)/t directed pairs",
+            "f32": {"value": rate32, "unit": "pair-interactions/s", "cores": cores, "kind": kind,
+                    "sample": f"the same with {what.format('_F32')}, first {m32} bodies, {sec32:.1f} s"}}
 
 
 def main():
@@ -330,7 +346,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and rank == 0:
         cpu = cpu_baseline(state)
-        log(f"cpu baseline {cpu['value']:.3e} pairs/s on {cpu['cores']} threads")
+        log(f"cpu baseline {cpu['value']:.3e} pairs/s (FP64), {cpu['f32']['value']:.3e} (_F32) "
+            f"on {cpu['cores']} threads, {cpu['kind']}")
 
     traffic = measured_traffic()
     out = {
