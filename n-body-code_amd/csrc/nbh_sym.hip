@@ -9,10 +9,11 @@
 //   * bodies are cut into blocks of B = 64*K*W (K targets per lane, W waves
 //     per workgroup; the last block may be partial, its missing lanes padded
 //     with massless bodies far away); block P's pairs with block Q != P are
-//     evaluated by one
-//     workgroup, for Q = P+1 .. P+H (mod nb), H = (nb-1)/2, plus Q = P+nb/2
-//     for P < nb/2 when nb is even, so every unordered block pair is taken
-//     exactly once and every block has the same work;
+//     evaluated by one workgroup, for Q = P+1 .. P+H (mod nb), H = (nb-1)/2,
+//     plus Q = P+nb/2 for P < nb/2 when nb is even, so every unordered block
+//     pair is taken exactly once and every block has the same work;
+//   * the ensemble drives the same body from a list of (system, block,
+//     block-pair range) items (k_ens_sym);
 //   * pairs inside a block are evaluated one-sided by the directed kernel
 //     (launch_force_blockdiag), which also initialises the outputs; these
 //     kernels then add their contributions with FP64 atomics;
@@ -53,7 +54,10 @@ constexpr int kSymTile = 64;  // sources per LDS tile (4 groups of 16)
 // own distance.  Every pair with a pad then has r^3 m = 0 exactly and all
 // intermediates finite (s ~ 1e36 fits FP32), so pads add nothing to real
 // bodies; their own sums are never written.
-__device__ __forceinline__ double pad_x(int k, bool source) { return (source ? -1e18 : 1e18) * (1.0 + 1e-3 * (k & 1023)); }
+__device__ __forceinline__ double pad_x(int k, bool source)
+{
+    return (source ? -1e18 : 1e18) * (1.0 + 1e-3 * (k & 1023));
+}
 
 // Row rotation by R lanes: lane k of each 16-lane row receives the value of
 // lane (k - R) mod 16 of the same row.
