@@ -294,9 +294,8 @@ int nbh_launch_shape(nbh_ctx* c, int n, int* block, int* jsplit, int* workgroups
     if (jsplit) *jsplit = s.jsplit;
     if (workgroups) {
         if (s.sym_chunk > 0) {
-            const int b = nbh::sym_block(c->cfg.precision);
-            const int nb = (n + b - 1) / b;
-            *workgroups = nb * (((nb - 1) / 2 + s.sym_chunk) / s.sym_chunk) + (n + 255) / 256;
+            // the symmetric grid, then the block-diagonal pass (256 targets per workgroup)
+            *workgroups = (int)(nbh::sym_workgroups(c->cfg.precision, n, s.sym_chunk) + (n + 255) / 256);
         } else {
             *workgroups = ((n + s.block * s.ipl - 1) / (s.block * s.ipl)) * s.jsplit;
         }

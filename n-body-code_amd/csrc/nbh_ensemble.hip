@@ -372,14 +372,13 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
         const int Q = ipl;
         const int per = nbh::kLanes * Q;
         if (e->sym && nbh::sym_supported(prec, n)) {
-            const int nb = (n + SB - 1) / SB;
-            const int half = (nb - 1) / 2;
+            const int nb = (n + SB - 1) / SB, nr = n / SB;
             for (int P = 0; P < nb; ++P) {
                 const int b0 = P * SB, b1 = std::min(n, b0 + SB);
                 for (int ib = b0; ib < b1; ib += per)
                     items.push_back({k, off + ib, std::min(per, b1 - ib), off + b0, off + b1, T, -1, Q});
-                const int dmax = half + ((nb % 2 == 0 && P < nb / 2) ? 1 : 0);
-                if (dmax >= 1) sym.push_back({off, n, nb, P, 1, dmax, e->eps2[k]});
+                const int len = P < nr ? nbh::sym_list_len(nr, P, nb > nr) : 0;
+                if (len >= 1) sym.push_back({off, n, P, 0, len - 1, e->eps2[k]});
             }
             continue;
         }
@@ -400,10 +399,17 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
             }
         }
     }
-    // most block pairs first
-    std::stable_sort(sym.begin(), sym.end(), [](const nbh::SymItem& a, const nbh::SymItem& b) {
-        return (long)(a.d1 - a.d0 + 1) * a.n > (long)(b.d1 - b.d0 + 1) * b.n;
-    });
+    // most pairs first: sources listed in the item's entries (the partial
+    // block, if listed, is the last entry of its list)
+    auto work = [SB](const nbh::SymItem& it) {
+        const int nr = it.n / SB;
+        const int full = nbh::sym_list_len(nr, it.P, false);  // entries before the partial block
+        double w = std::min(it.i1 + 1, full) - it.i0;
+        if (it.i1 >= full) w += (double)(it.n - nr * SB) / SB;
+        return w;
+    };
+    std::stable_sort(sym.begin(), sym.end(),
+                     [&](const nbh::SymItem& a, const nbh::SymItem& b) { return work(a) > work(b); });
     // largest source range first; stable keeps system order among equals
     std::stable_sort(items.begin(), items.end(),
                      [](const EnsItem& a, const EnsItem& b) { return (a.j1 - a.j0) > (b.j1 - b.j0); });

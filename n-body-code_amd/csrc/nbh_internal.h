@@ -56,17 +56,26 @@ hipError_t launch_force_blockdiag(int precision, const State& s, int n, double e
 // both launches.
 int sym_block(int precision);
 bool sym_supported(int precision, int n);
+// Entries in the source list of full block P of nr (the last entry is the
+// partial block, if there is one).
+__host__ __device__ inline int sym_list_len(int nr, int P, bool partial)
+{
+    return (nr - 1) / 2 + ((nr % 2 == 0 && P < nr / 2) ? 1 : 0) + (partial ? 1 : 0);
+}
+int sym_list_max(int precision, int n);  // longest source list of a full block
 int sym_chunk(int precision, int n, int cus);
+long sym_workgroups(int precision, int n, int chunk);
 hipError_t launch_force_sym(int precision, const State& s, int n, double eps2,
                             int chunk, const Forces& out, hipStream_t st,
                             hipEvent_t ev0, hipEvent_t ev1);
 
-// One work item of the list-driven symmetric kernel (the ensemble): block P
-// of the system occupying [off, off + n) against blocks P + d0 .. P + d1
-// (mod nb, d0 >= 1) of the same system, with its eps2.  The outputs must
-// already hold the block-diagonal sums: the items add to them atomically.
+// One work item of the list-driven symmetric kernel (the ensemble): full
+// block P of the system occupying [off, off + n) against entries i0 .. i1 of
+// its source list (blocks P+1 .. P+H mod nr, then the partial block if any),
+// with the system's eps2.  The outputs must already hold the block-diagonal
+// sums: the items add to them atomically.
 struct SymItem {
-    int off, n, nb, P, d0, d1;
+    int off, n, P, i0, i1;
     double eps2;
 };
 hipError_t launch_ens_sym(int precision, const State& s, const Forces& out,

@@ -6,14 +6,18 @@
 // FP64 (the directed FP64 kernel's arithmetic).
 //
 // Layout of the work:
-//   * bodies are cut into blocks of B = 64*K*W (K targets per lane, W waves
-//     per workgroup; the last block may be partial, its missing lanes padded
-//     with massless bodies far away); block P's pairs with block Q != P are
-//     evaluated by one workgroup, for Q = P+1 .. P+H (mod nb), H = (nb-1)/2,
-//     plus Q = P+nb/2 for P < nb/2 when nb is even, so every unordered block
-//     pair is taken exactly once and every block has the same work;
-//   * the ensemble drives the same body from a list of (system, block,
-//     block-pair range) items (k_ens_sym);
+//   * bodies are cut into nr = n / B full blocks of B = 64*K*W (K targets per
+//     lane, W waves per workgroup) and, when B does not divide n, a partial
+//     block nr of the n - nr*B remaining bodies.  Full block P is the target
+//     block of the pairs with its source list: blocks P+1 .. P+H (mod nr),
+//     H = (nr-1)/2, plus P+nr/2 for P < nr/2 when nr is even, then the
+//     partial block.  Every unordered block pair is taken exactly once and
+//     every full block has the same work; the partial block is only ever a
+//     source, and its tiles past n are not visited (the last visited one is
+//     padded with massless bodies far away);
+//   * a workgroup takes `chunk` consecutive entries of one source list; the
+//     ensemble drives the same body from a list of (system, block, entry
+//     range) items (k_ens_sym);
 //   * pairs inside a block are evaluated one-sided by the directed kernel
 //     (launch_force_blockdiag), which also initialises the outputs; these
 //     kernels then add their contributions with FP64 atomics;
@@ -49,14 +53,14 @@ namespace {
 
 constexpr int kSymTile = 64;  // sources per LDS tile (4 groups of 16)
 
-// Lanes past the last body (the partial last block) hold pad bodies: zero
-// mass, at rest, 1e18 away along +x (targets) or -x (sources), each at its
-// own distance.  Every pair with a pad then has r^3 m = 0 exactly and all
+// Source records past the last body (in the last tile of the partial block)
+// hold pad bodies: zero mass, at rest, 1e18 away along -x, each at its own
+// distance.  Every pair with a pad then has r^3 m = 0 exactly and all
 // intermediates finite (s ~ 1e36 fits FP32), so pads add nothing to real
 // bodies; their own sums are never written.
-__device__ __forceinline__ double pad_x(int k, bool source)
+__device__ __forceinline__ double pad_x(int j)
 {
-    return (source ? -1e18 : 1e18) * (1.0 + 1e-3 * (k & 1023));
+    return -1e18 * (1.0 + 1e-3 * (j & 1023));
 }
 
 // Row rotation by R lanes: lane k of each 16-lane row receives the value of
@@ -101,22 +105,17 @@ struct SymMixed {
     {
         if (j < n) return load_jbmx(s, j);
         Rec b{};
-        b.x = pad_x(j, true);
+        b.x = pad_x(j);
         return b;
     }
-    static __device__ __forceinline__ void load_targets(const State& s, int ibase, int n, Targets& I)
+    static __device__ __forceinline__ void load_targets(const State& s, int ibase, Targets& I)
     {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = ibase + k * 64;
-            const int ic = i < n ? i : n - 1;
-            I.x[k] = s.x[ic]; I.y[k] = s.y[ic]; I.z[k] = s.z[ic];
-            I.m[k] = (float)s.m[ic];
-            I.vx[k] = (float)s.vx[ic]; I.vy[k] = (float)s.vy[ic]; I.vz[k] = (float)s.vz[ic];
-            if (i >= n) {
-                I.x[k] = pad_x(i, false); I.y[k] = I.z[k] = 0.0;
-                I.m[k] = I.vx[k] = I.vy[k] = I.vz[k] = 0.f;
-            }
+            I.x[k] = s.x[i]; I.y[k] = s.y[i]; I.z[k] = s.z[i];
+            I.m[k] = (float)s.m[i];
+            I.vx[k] = (float)s.vx[i]; I.vy[k] = (float)s.vy[i]; I.vz[k] = (float)s.vz[i];
         }
     }
     template <int R>
@@ -210,23 +209,18 @@ struct SymF64 {
             b.x = s.x[j]; b.y = s.y[j]; b.z = s.z[j]; b.m = s.m[j];
             b.vx = s.vx[j]; b.vy = s.vy[j]; b.vz = s.vz[j];
         } else {
-            b.x = pad_x(j, true);
+            b.x = pad_x(j);
         }
         return b;
     }
-    static __device__ __forceinline__ void load_targets(const State& s, int ibase, int n, Targets& I)
+    static __device__ __forceinline__ void load_targets(const State& s, int ibase, Targets& I)
     {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = ibase + k * 64;
-            const int ic = i < n ? i : n - 1;
-            I.x[k] = s.x[ic]; I.y[k] = s.y[ic]; I.z[k] = s.z[ic];
-            I.m[k] = s.m[ic];
-            I.vx[k] = s.vx[ic]; I.vy[k] = s.vy[ic]; I.vz[k] = s.vz[ic];
-            if (i >= n) {
-                I.x[k] = pad_x(i, false); I.y[k] = I.z[k] = 0.0;
-                I.m[k] = I.vx[k] = I.vy[k] = I.vz[k] = 0.0;
-            }
+            I.x[k] = s.x[i]; I.y[k] = s.y[i]; I.z[k] = s.z[i];
+            I.m[k] = s.m[i];
+            I.vx[k] = s.vx[i]; I.vy[k] = s.vy[i]; I.vz[k] = s.vz[i];
         }
     }
     template <int R>
@@ -298,19 +292,17 @@ __device__ __forceinline__ void sym_group(const typename T::Rec* __restrict__ ro
 
 struct SymArgs {
     State s;
-    int n;        // bodies
-    int nb;       // blocks of 64*K*W bodies, the last one possibly partial
-    int half;     // H = (nb - 1) / 2
-    int chunk;    // block pairs per workgroup
+    int n;        // bodies, >= 2 full blocks
+    int chunk;    // source-list entries per workgroup
     double eps2;
     Forces out;   // initialised by the block-diagonal pass
 };
 
-// Block P of one system (bodies 0..n-1 of s and out) against blocks
-// P + d0 .. P + d1 (mod nb), d0 >= 1: the body of both symmetric kernels.
+// Full block P of one system (bodies 0..n-1 of s and out) against entries
+// i0 .. i1 of its source list: the body of both symmetric kernels.
 template <template <int> class TT, int K, int W>
-__device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& out, int n, int nb, double eps2d,
-                                                int P, int d0, int d1)
+__device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& out, int n, double eps2d, int P,
+                                                int i0, int i1)
 {
     using T = TT<K>;
     using Sum = typename T::Sum;
@@ -318,7 +310,10 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
     constexpr int TPB = B / kSymTile;  // tiles per block
     __shared__ typename T::Rec dup[2][4][32];
     __shared__ Sum part[2][W][kSymTile][6];
-    const int ntiles = (d1 - d0 + 1) * TPB;
+    const int nr = n / B;
+    const int dmax = sym_list_len(nr, P, false);  // entry dmax, if listed, is the partial block
+    const int vlast = (n - nr * B + kSymTile - 1) / kSymTile;  // its tiles holding bodies
+    const int ntiles = i1 < dmax ? (i1 - i0 + 1) * TPB : (i1 - i0) * TPB + vlast;
 
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -328,11 +323,13 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
     const int ibase = P * B + w * 64 * K + lane;
 
     typename T::Targets I;
-    T::load_targets(s, ibase, n, I);
+    T::load_targets(s, ibase, I);
     // first source of tile tt of this workgroup's range
     auto tile_src = [&](int tt) {
-        int q = P + d0 + tt / TPB;
-        if (q >= nb) q -= nb;
+        const int i = i0 + tt / TPB;
+        int q = P + 1 + i;
+        if (q >= nr) q -= nr;
+        if (i == dmax) q = nr;
         return q * B + (tt % TPB) * kSymTile;
     };
 
@@ -394,8 +391,7 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-            if (ibase + k * 64 < n) unsafeAtomicAdd(&out.c[c][ibase + k * 64], T::total(tot, k, c));
+        for (int c = 0; c < 6; ++c) unsafeAtomicAdd(&out.c[c][ibase + k * 64], T::total(tot, k, c));
 }
 
 // Resident waves per SIMD asked of the compiler: T::kWaves (mixed 5: 96
@@ -406,16 +402,17 @@ template <template <int> class TT, int K, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::kWaves, 8))) void k_force_sym(
     SymArgs a)
 {
+    constexpr int B = 64 * K * W;
     const int P = blockIdx.x;
-    const int dmax = a.half + ((a.nb % 2 == 0 && P < a.nb / 2) ? 1 : 0);
-    const int d0 = 1 + blockIdx.y * a.chunk;
-    if (d0 > dmax) return;  // uniform over the workgroup, before any barrier
-    const int d1 = min(dmax, d0 + a.chunk - 1);
-    sym_block_pairs<TT, K, W>(a.s, a.out, a.n, a.nb, a.eps2, P, d0, d1);
+    const int len = sym_list_len(a.n / B, P, a.n % B != 0);
+    const int i0 = blockIdx.y * a.chunk;
+    if (i0 >= len) return;  // uniform over the workgroup, before any barrier
+    const int i1 = min(len, i0 + a.chunk) - 1;
+    sym_block_pairs<TT, K, W>(a.s, a.out, a.n, a.eps2, P, i0, i1);
 }
 
 // Many systems: one workgroup per item of a list (system offset and size,
-// block P and its block-pair range), systems' arrays concatenated.
+// block P and a range of its source list), systems' arrays concatenated.
 template <template <int> class TT, int K, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::kWaves, 8))) void k_ens_sym(
     State s, Forces out, const SymItem* __restrict__ items)
@@ -427,7 +424,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
     Forces o = out;
 #pragma unroll
     for (int c = 0; c < 6; ++c) o.c[c] += it.off;
-    sym_block_pairs<TT, K, W>(q, o, it.n, it.nb, it.eps2, it.P, it.d0, it.d1);
+    sym_block_pairs<TT, K, W>(q, o, it.n, it.eps2, it.P, it.i0, it.i1);
 }
 
 // Launch shape per precision: targets per lane, waves per workgroup.
@@ -440,19 +437,30 @@ int sym_block(int precision) { return precision == 0 ? 64 * kF64K * kF64W : 64 *
 
 bool sym_supported(int precision, int n) { return n >= 2 * sym_block(precision); }
 
+int sym_list_max(int precision, int n)
+{
+    const int b = sym_block(precision);
+    return sym_list_len(n / b, 0, n % b != 0);
+}
+
 int sym_chunk(int precision, int n, int cus)
 {
     const int b = sym_block(precision);
-    const int nb = (n + b - 1) / b;
-    const int pairs_per_block = (nb - 1) / 2 + 1;
+    const int per = sym_list_max(precision, n);
     // about 64 workgroups per CU in all (some 16 rounds of resident ones),
     // so the last round is a small part; measured at n = 2^18 and 2^20
     // against 8..256: within 1-3% of the best
     const long want = 64L * (cus > 0 ? cus : 256);
-    long c = ((long)nb * pairs_per_block + want - 1) / want;
+    long c = ((long)(n / b) * per + want - 1) / want;
     if (c < 1) c = 1;
-    if (c > pairs_per_block) c = pairs_per_block;
+    if (c > per) c = per;
     return (int)c;
+}
+
+long sym_workgroups(int precision, int n, int chunk)
+{
+    const int per = sym_list_max(precision, n);
+    return (long)(n / sym_block(precision)) * ((per + chunk - 1) / chunk);
 }
 
 hipError_t launch_ens_sym(int precision, const State& s, const Forces& out, const SymItem* items, int nitems,
@@ -477,13 +485,11 @@ hipError_t launch_force_sym(int precision, const State& s, int n, double eps2, i
     SymArgs a;
     a.s = s;
     a.n = n;
-    a.nb = (n + B - 1) / B;
-    a.half = (a.nb - 1) / 2;
     a.chunk = chunk;
     a.eps2 = eps2;
     a.out = out;
-    const int per = a.half + 1;
-    const dim3 grid(a.nb, (per + chunk - 1) / chunk);
+    const int per = sym_list_max(precision, n);
+    const dim3 grid(n / B, (per + chunk - 1) / chunk);
     if (precision == 0)
         k_force_sym<SymF64, kF64K, kF64W><<<grid, 64 * kF64W, 0, st>>>(a);
     else

@@ -201,8 +201,9 @@ def test_bad_arguments(nb):
 
 
 # ---- symmetric pairing (context created with pairing="symmetric"): systems
-# of >= 1024 bodies evaluate each unordered pair once
-SYM_SIZES = [1024, 1025, 1500, 2048, 3001, 600, 100, 5, 1]
+# of >= 1024 bodies evaluate each unordered pair once; 1025 .. 3001 end in a
+# partial block of 1, 76, 200, 300, 476 and 441 bodies
+SYM_SIZES = [1024, 1025, 1100, 1224, 1324, 1500, 2048, 3001, 600, 100, 5, 1]
 
 
 @pytest.fixture(scope="module")

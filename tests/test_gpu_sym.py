@@ -26,9 +26,11 @@ def force(nb, state, eps2, pairing, precision="mixed"):
 
 
 # n = 1024 .. 2560: 2, 3, 4, 5 blocks of 512 (even and odd ring); 1025, 1100,
-# 2500, 3001: a partial last block (1, 76, 452, 441 bodies) padded inside
+# 1224, 1324, 2500, 3001: a partial last block (1, 76, 200, 300, 452, 441
+# bodies), a source block only, of which 1, 2, 4, 5, 8, 7 of 8 tiles are
+# visited, the last one padded inside
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-@pytest.mark.parametrize("n", [1024, 1536, 2048, 2560, 1025, 1100, 2500, 3001])
+@pytest.mark.parametrize("n", [1024, 1536, 2048, 2560, 1025, 1100, 1224, 1324, 2500, 3001])
 def test_symmetric_matches_oracle(nb, oracle, n, precision):
     _banner(f"oracle {precision} n={n}")
     s = nb.plummer(n, 11 + n)
@@ -59,6 +61,35 @@ def test_symmetric_close_to_directed(nb, precision):
     _banner(f"{precision}: acc median {np.median(ea):.2e} max {ea.max():.2e}; "
             f"jerk median {np.median(ej):.2e} max {ej.max():.2e}")
     am, ax, jm, jx = TOL[precision]
+    assert np.median(ea) <= am and np.median(ej) <= jm
+    if precision == "fp64":
+        assert ea.max() <= 1e-12 and ej.max() <= 1e-12
+
+
+# 399 full blocks and a partial one of 100 bodies: workgroups take up to 5
+# entries of a block's source list, which ends with the partial block
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_partial_block_inside_workgroup_ranges(nb, oracle, precision):
+    n = 399 * 512 + 100
+    _banner(f"partial block, multi-pair workgroups {precision} n={n}")
+    s = nb.plummer(n, 23)
+    eps2 = (1 / 256) ** 2
+    fs, shape = force(nb, s, eps2, "symmetric", precision)
+    assert shape["pairing"] == "symmetric"
+    fd, _ = force(nb, s, eps2, "directed", precision)
+    ea = body_relerr(fs[:3], fd[:3])
+    ej = body_relerr(fs[3:], fd[3:])
+    _banner(f"{precision} vs directed: acc median {np.median(ea):.2e} max {ea.max():.2e}; "
+            f"jerk median {np.median(ej):.2e} max {ej.max():.2e}")
+    # a sample from every block kind: the partial one, its neighbours, the first
+    idx = np.unique(np.concatenate([np.arange(n - 100, n), np.arange(n - 612, n - 100, 8),
+                                    np.arange(0, 512, 8), np.random.default_rng(9).choice(n, 64)]))
+    ref = oracle.force_sample(s, eps2, idx, nthreads=16)
+    eo = body_relerr(fs[:3, idx], ref[:3])
+    eoj = body_relerr(fs[3:, idx], ref[3:])
+    am, ax, jm, jx = TOL[precision]
+    assert np.median(eo) <= am and eo.max() <= ax
+    assert np.median(eoj) <= jm and eoj.max() <= jx
     assert np.median(ea) <= am and np.median(ej) <= jm
     if precision == "fp64":
         assert ea.max() <= 1e-12 and ej.max() <= 1e-12
