@@ -201,10 +201,11 @@ int nbh_device_info(int device, int *compute_units, int *wave_size,
  * j-split chosen for that bucket, and all items one or two targets per lane
  * (by nbh_ens_autotune or nbh_ens_set_tuning / nbh_ens_set_targets_per_lane).
  * Precision and device come from the context.  When the context was created
- * with NBH_PAIR_SYMMETRIC, systems of >= 1024 bodies are evaluated with
- * symmetric pairing instead: the persistent launch covers only their
- * diagonal 512-body blocks, and a second launch adds every block pair once
- * (FP64 atomics, as in nbh_force).  The step is the shared-dt Hermite step
+ * with NBH_PAIR_SYMMETRIC, systems of at least two blocks (blocks of 256
+ * bodies in mixed precision, 512 in FP64) are evaluated with symmetric
+ * pairing instead: the persistent launch covers only their diagonal blocks,
+ * and a second launch adds every block pair once (FP64 atomics, as in
+ * nbh_force).  The step is the shared-dt Hermite step
  * of nbh_step applied to every system. */
 
 typedef struct nbh_ensemble nbh_ensemble;

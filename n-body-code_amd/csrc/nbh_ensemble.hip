@@ -363,7 +363,7 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
     sym.clear();
     nslots = 0;
     const int prec = e->ctx->cfg.precision;
-    const int SB = nbh::sym_block(prec);
+    const int SB = nbh::ens_sym_block(prec);
     for (int k = 0; k < e->nsys; ++k) {
         if (!use.empty() && !use[k]) continue;
         const int n = e->cnt[k], off = e->off[k];
@@ -371,7 +371,7 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
         const int T = tile[b];
         const int Q = ipl;
         const int per = nbh::kLanes * Q;
-        if (e->sym && nbh::sym_supported(prec, n)) {
+        if (e->sym && nbh::ens_sym_supported(prec, n)) {
             const int nb = (n + SB - 1) / SB, nr = n / SB;
             for (int P = 0; P < nb; ++P) {
                 const int b0 = P * SB, b1 = std::min(n, b0 + SB);

@@ -69,6 +69,10 @@ hipError_t launch_force_sym(int precision, const State& s, int n, double eps2,
                             int chunk, const Forces& out, hipStream_t st,
                             hipEvent_t ev0, hipEvent_t ev1);
 
+// The ensemble's symmetric kernel cuts systems into blocks of its own size.
+int ens_sym_block(int precision);
+bool ens_sym_supported(int precision, int n);
+
 // One work item of the list-driven symmetric kernel (the ensemble): full
 // block P of the system occupying [off, off + n) against entries i0 .. i1 of
 // its source list (blocks P+1 .. P+H mod nr, then the partial block if any),

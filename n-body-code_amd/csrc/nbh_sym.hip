@@ -395,11 +395,21 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
 }
 
 // Resident waves per SIMD asked of the compiler: T::kWaves (mixed 5: 96
-// VGPRs, a few spills; FP64 3).  gfx950 issues FP32 at twice the one-wave
-// rate only by pairing instructions of two waves, so more resident waves
-// raise the dual-issue share (mixed: +1.6% at n = 2^20 over 4 waves).
+// VGPRs, a few spills; FP64 3), or fewer where the workgroups' LDS (160 KiB
+// per CU) holds fewer (mixed with 2 waves: 18 KiB, 8 workgroups, 4 waves).
+// gfx950 issues FP32 at twice the one-wave rate only by pairing instructions
+// of two waves, so more resident waves raise the dual-issue share (mixed:
+// +1.6% at n = 2^20 over 4 waves).
+template <class T, int W>
+constexpr int sym_waves()
+{
+    constexpr int lds = 2 * 4 * 32 * sizeof(typename T::Rec) + 2 * W * kSymTile * 6 * sizeof(typename T::Sum);
+    constexpr int by_lds = (160 * 1024 / lds) * W / 4;
+    return by_lds < T::kWaves ? by_lds : T::kWaves;
+}
+
 template <template <int> class TT, int K, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::kWaves, 8))) void k_force_sym(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_waves<TT<K>, W>(), 8))) void k_force_sym(
     SymArgs a)
 {
     constexpr int B = 64 * K * W;
@@ -414,7 +424,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
 // Many systems: one workgroup per item of a list (system offset and size,
 // block P and a range of its source list), systems' arrays concatenated.
 template <template <int> class TT, int K, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::kWaves, 8))) void k_ens_sym(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_waves<TT<K>, W>(), 8))) void k_ens_sym(
     State s, Forces out, const SymItem* __restrict__ items)
 {
     const SymItem it = items[blockIdx.x];
@@ -427,15 +437,21 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TT<K>::k
     sym_block_pairs<TT, K, W>(q, o, it.n, it.eps2, it.P, it.i0, it.i1);
 }
 
-// Launch shape per precision: targets per lane, waves per workgroup.
-constexpr int kMixK = 2, kMixW = 4;
-constexpr int kF64K = 2, kF64W = 4;
+// Launch shape per precision: targets per lane, waves per workgroup (one
+// system; the ensemble's, whose diagonal blocks are a larger share of the
+// pairs).
+constexpr int kMixK = 2, kMixW = 4, kMixEnsW = 2;
+constexpr int kF64K = 2, kF64W = 4, kF64EnsW = 4;
 
 }  // namespace
 
 int sym_block(int precision) { return precision == 0 ? 64 * kF64K * kF64W : 64 * kMixK * kMixW; }
 
 bool sym_supported(int precision, int n) { return n >= 2 * sym_block(precision); }
+
+int ens_sym_block(int precision) { return precision == 0 ? 64 * kF64K * kF64EnsW : 64 * kMixK * kMixEnsW; }
+
+bool ens_sym_supported(int precision, int n) { return n >= 2 * ens_sym_block(precision); }
 
 int sym_list_max(int precision, int n)
 {
@@ -468,9 +484,9 @@ hipError_t launch_ens_sym(int precision, const State& s, const Forces& out, cons
 {
     if (nitems <= 0) return hipSuccess;
     if (precision == 0)
-        k_ens_sym<SymF64, kF64K, kF64W><<<nitems, 64 * kF64W, 0, st>>>(s, out, items);
+        k_ens_sym<SymF64, kF64K, kF64EnsW><<<nitems, 64 * kF64EnsW, 0, st>>>(s, out, items);
     else
-        k_ens_sym<SymMixed, kMixK, kMixW><<<nitems, 64 * kMixW, 0, st>>>(s, out, items);
+        k_ens_sym<SymMixed, kMixK, kMixEnsW><<<nitems, 64 * kMixEnsW, 0, st>>>(s, out, items);
     return hipGetLastError();
 }
 

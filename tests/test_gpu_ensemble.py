@@ -201,9 +201,10 @@ def test_bad_arguments(nb):
 
 
 # ---- symmetric pairing (context created with pairing="symmetric"): systems
-# of >= 1024 bodies evaluate each unordered pair once; 1025 .. 3001 end in a
-# partial block of 1, 76, 200, 300, 476 and 441 bodies
-SYM_SIZES = [1024, 1025, 1100, 1224, 1324, 1500, 2048, 3001, 600, 100, 5, 1]
+# of >= 2 blocks (256 bodies in mixed precision, 512 in FP64) evaluate each
+# unordered pair once; sizes with and without a partial last block, either
+# side of both thresholds
+SYM_SIZES = [1024, 1025, 1100, 1224, 1324, 1500, 2048, 3001, 600, 511, 512, 513, 767, 100, 5, 1]
 
 
 @pytest.fixture(scope="module")
