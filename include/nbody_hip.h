@@ -60,8 +60,10 @@ typedef struct nbh_config {
     int precision;  /* enum nbh_precision */
     double eps2;    /* Plummer softening squared; the reference hard-codes
                        EPS2 = 1e-7 (include/kernel.h:13) */
-    int block;      /* i-bodies (= threads) per workgroup and j-tile length:
-                       64, 128 or 256; 0 = pick by size */
+    int block;      /* i-bodies (= threads) per workgroup and, in FP64, the
+                       LDS j-tile length: 64, 128 or 256; 0 = pick by size
+                       (the mixed kernel reads sources through scalar loads
+                       and has no tile) */
     int jsplit;     /* j-range split across workgroups; 0 = pick by size */
     int ipl;        /* targets per lane, 1 or 2; 0 = pick by size */
     int pairing;    /* enum nbh_pairing; 0 = NBH_PAIR_DIRECTED.  block,
@@ -179,7 +181,8 @@ int nbh_launch_pairing(nbh_ctx *ctx, int n, int *pairing);
  * per lane {1,2} x j-split {1..32} for n bodies on this context's device and
  * precision (a Plummer sphere, on a source range of max(min(n, 65536), n/16)
  * bodies so that a trial is short; when that range is truncated, the three
- * fastest shapes are timed again on all n sources) and keeps the fastest
+ * fastest shapes, and the fastest one with half and twice its j-split, are
+ * timed again on all n sources) and keeps the fastest
  * for n's size bucket (floor(log2 n)) and precision.  Later evaluations in
  * that bucket use it unless the config forces block/jsplit/ipl.  Directed
  * pairing only: the symmetric kernel has a fixed shape.  Outputs may be

@@ -283,7 +283,7 @@ int adj_force(nbh_adjoint* a, double* st, double* f)
     sh.jchunk = per * 64;
     sh.jsplit = (n + sh.jchunk - 1) / sh.jchunk;
     hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(st, n), n, c->cfg.eps2, sh, nbh::forces_of(f, n),
-                                     a->partial.d(), c->stream, nullptr, nullptr);
+                                     a->partial.d(), nullptr, c->stream, nullptr, nullptr);
     if (e != hipSuccess) return fail(NBH_ERR_HIP, "adjoint force launch failed: %s", hipGetErrorString(e));
     return NBH_OK;
 }

@@ -180,13 +180,14 @@ int enqueue_force(nbh_ctx* c, const nbh::State& s, int n, const nbh::Forces& out
         NBH_HIP(c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n));
         part = c->partial.d();
     }
+    NBH_HIP(c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)));
     EventPair ep;
     const bool timed = c->profiling && allow_events;
     if (timed) {
         int rc = get_events(c, &ep);
         if (rc) return rc;
     }
-    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, out, part, st,
+    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, out, part, c->rec.p, st,
                                      timed ? ep.a : nullptr, timed ? ep.b : nullptr);
     if (e != hipSuccess) return fail(NBH_ERR_HIP, "force launch failed: %s", hipGetErrorString(e));
     if (timed) c->ev_pending.push_back(ep);
@@ -265,7 +266,7 @@ int nbh_destroy(nbh_ctx* c)
         (void)hipEventDestroy(ep.a);
         (void)hipEventDestroy(ep.b);
     }
-    for (DevBuf* b : {&c->in, &c->out, &c->partial, &c->flag, &c->scal, &c->state, &c->f0,
+    for (DevBuf* b : {&c->in, &c->out, &c->partial, &c->rec, &c->flag, &c->scal, &c->state, &c->f0,
                       &c->f1, &c->ework})
         b->release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -349,11 +350,13 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
     auto time_shape = [&](const nbh::Shape& sh, int reps, int trial_nj, double* out_ms) -> int {
         if (sh.jsplit > 1 && c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n) != hipSuccess)
             return fail(NBH_ERR_ALLOC, "autotune: partial-sum allocation failed");
+        if (c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)) != hipSuccess)
+            return fail(NBH_ERR_ALLOC, "autotune: source-record allocation failed");
         for (int rep = 0; rep < reps; ++rep) {
             hipError_t e = hipEventRecord(e0, c->stream);
             if (e == hipSuccess)
-                e = nbh::launch_force(c->cfg.precision, st, n, c->cfg.eps2, sh, fo, c->partial.d(), c->stream,
-                                      nullptr, nullptr, trial_nj);
+                e = nbh::launch_force(c->cfg.precision, st, n, c->cfg.eps2, sh, fo, c->partial.d(), c->rec.p,
+                                      c->stream, nullptr, nullptr, trial_nj);
             if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;
@@ -385,9 +388,20 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
     if (rc == NBH_OK) {
         std::sort(trials.begin(), trials.end(), [](const Trial& a, const Trial& b) { return a.ms < b.ms; });
         // stage 2 (truncated trials only): the three fastest again on all n
-        // sources, since the truncation changes how the splits balance
+        // sources, since the truncation changes how the splits balance, and
+        // the fastest one's shape with half and twice its split
         const bool full = nj < n;
-        const size_t top = full ? std::min<size_t>(3, trials.size()) : std::min<size_t>(1, trials.size());
+        size_t top = full ? std::min<size_t>(3, trials.size()) : std::min<size_t>(1, trials.size());
+        if (full && !trials.empty()) {
+            const Trial b0 = trials[0];
+            for (int sp : {b0.split / 2, b0.split * 2}) {
+                if (sp < 1 || sp > 32) continue;
+                bool have = false;
+                for (size_t k = 0; k < top; ++k)
+                    have |= trials[k].block == b0.block && trials[k].ipl == b0.ipl && trials[k].split == sp;
+                if (!have) trials.insert(trials.begin() + top++, Trial{0.0, b0.block, b0.ipl, sp});
+            }
+        }
         for (size_t k = 0; k < top && rc == NBH_OK; ++k) {
             double ms = trials[k].ms;
             if (full) rc = time_shape(make_shape(n, trials[k].block, trials[k].ipl, trials[k].split), 1, 0, &ms);
@@ -446,8 +460,10 @@ int nbh_force(nbh_ctx* c, int n, const double* x, const double* y, const double*
             return fail(NBH_ERR_ALLOC, "device allocation of split partials failed");
         part = c->partial.d();
     }
+    if (c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)) != hipSuccess)
+        return fail(NBH_ERR_ALLOC, "device allocation of source records failed");
     NBH_HIP(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
-    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, f, part, c->stream,
+    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, f, part, c->rec.p, c->stream,
                                      ep.a, ep.b);
     if (e != hipSuccess) {
         c->ev_free.push_back(ep);

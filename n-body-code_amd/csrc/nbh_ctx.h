@@ -61,6 +61,7 @@ struct nbh_ctx {
     nbh::DevBuf in;       // 7n
     nbh::DevBuf out;      // 6n
     nbh::DevBuf partial;  // 6 * jsplit * n
+    nbh::DevBuf rec;      // mixed: packed source records, 48 B per body
     nbh::DevBuf flag;     // int
     nbh::DevBuf scal;     // scalars: [0] dt, [1] dt_next bits, [2..3] energy, [4..] spare
 

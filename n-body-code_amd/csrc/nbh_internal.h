@@ -39,16 +39,21 @@ struct Shape {
 // launches).  When shape.jsplit > 1, `partial` must hold 6*jsplit*n doubles
 // and the split-reduction kernel is launched after the pair kernel.  ev0/ev1
 // (may be null) are recorded around the pair kernel only.
+// rec: device buffer of source_record_bytes(precision, n) bytes for the
+// mixed kernels' packed source records (unused, may be null, in FP64).
+size_t source_record_bytes(int precision, int n);
+// Packs sources 0..n-1 of s into rec (mixed; nothing to do in FP64).
+hipError_t pack_source_records(int precision, const State& s, int n, void* rec, hipStream_t st);
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out,
-                        double* partial, hipStream_t st, hipEvent_t ev0,
+                        double* partial, void* rec, hipStream_t st, hipEvent_t ev0,
                         hipEvent_t ev1, int nj = 0);
 
 // Directed pair kernel restricted to the diagonal blocks: target i gets the
 // sources of its own block [B*(i/B), B*(i/B)+B), self-pair excluded; the
 // outputs are overwritten (the last block may be partial).  B % 256 == 0.
 hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
-                                  const Forces& out, hipStream_t st);
+                                  const Forces& out, void* rec, hipStream_t st);
 
 // Symmetric pair kernels (nbh_sym.hip): block-diagonal pass, then every
 // off-diagonal block pair once with the reaction added to the source side.
@@ -66,7 +71,7 @@ int sym_list_max(int precision, int n);  // longest source list of a full block
 int sym_chunk(int precision, int n, int cus);
 long sym_workgroups(int precision, int n, int chunk);
 hipError_t launch_force_sym(int precision, const State& s, int n, double eps2,
-                            int chunk, const Forces& out, hipStream_t st,
+                            int chunk, const Forces& out, void* rec, hipStream_t st,
                             hipEvent_t ev0, hipEvent_t ev1);
 
 // The ensemble's symmetric kernel cuts systems into blocks of its own size.

@@ -3,13 +3,20 @@
 // Pair kernels (the hot path, replacing the reference's force loops
 // src/kernel.cpp:1101-1224 and the Tensix nb_compute pipeline
 // kernels/nb_compute.cpp:180-582):
-//   * one target (i) body per lane, BLOCK lanes per workgroup;
-//   * source (j) bodies streamed from HBM in tiles of BLOCK bodies: each lane
-//     loads one body with coalesced SoA loads and stores it into an
-//     LDS tile (array-of-structs, 16-byte aligned); two tiles alternate so the
-//     global loads of tile t+1 overlap the arithmetic on tile t;
-//   * the inner loop reads tile entries at a wave-uniform address, so every
-//     LDS read is a broadcast to the 64 lanes;
+//   * one or two target (i) bodies per lane, BLOCK lanes per workgroup;
+//   * every lane of a wave takes the same source (j) body at the same time,
+//     so the source record is wave-uniform:
+//       - FP64: sources streamed from HBM in tiles of BLOCK bodies, each lane
+//         loading one body with coalesced SoA loads into an LDS tile
+//         (array-of-structs, 16-byte aligned; two tiles alternate so the
+//         loads of tile t+1 overlap the arithmetic on tile t), and read back
+//         at a wave-uniform address, a broadcast to the 64 lanes;
+//       - mixed: sources packed once per evaluation into 48-byte records
+//         (FP64 position, FP32 velocity and mass) and read with scalar loads
+//         into SGPRs, which the VALU instructions take as operands: no LDS,
+//         no barriers, and the registers of a record are scalar, so twice as
+//         many waves stay resident (FP32 reaches its rate only by pairing
+//         instructions of two waves);
 //   * grid.y splits the j-range when n is too small to fill the CUs; the
 //     partial sums are added by a second kernel in split order, so results
 //     are deterministic run to run;
@@ -123,12 +130,20 @@ __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
     }
 }
 
-template <int BLOCK, int IPL, bool MASK>
-__global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a)
+// Mixed source records, packed from the SoA state before each evaluation.
+__global__ __launch_bounds__(kOBlock) void k_pack_mixed(State s, int n, JBMX* __restrict__ rec)
 {
-    static_assert(BLOCK % kFold == 0, "tile must hold whole fold runs");
-    constexpr int UNROLL = IPL == 1 ? 4 : 2;
-    __shared__ JBMX tile[2][BLOCK];
+    const int j = blockIdx.x * kOBlock + threadIdx.x;
+    if (j < n) rec[j] = load_jbmx(s, j);
+}
+
+// Sources from `rec` (k_pack_mixed) at wave-uniform indices: scalar loads.
+// Terms are summed over runs of kFold sources starting at the split's first
+// source, whatever BLOCK and IPL, then folded into the (hi, lo) totals.
+template <int BLOCK, int IPL, bool MASK>
+__global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a, const JBMX* __restrict__ rec)
+{
+    constexpr int UNROLL = IPL == 1 ? 8 : 4;
     const int t = threadIdx.x;
     const int base = blockIdx.x * BLOCK * IPL + t;
     const float eps2 = (float)a.eps2;
@@ -147,48 +162,34 @@ __global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a)
 
     int jbeg, jend;
     source_range<BLOCK, IPL>(a, jbeg, jend);
-    JBMX st{};
-    if (jbeg + t < jend) st = load_jbmx(a.s, jbeg + t);
-    tile[0][t] = st;
-    __syncthreads();
-    int buf = 0;
-    for (int jt = jbeg; jt < jend; jt += BLOCK) {
-        const int cnt = min(BLOCK, jend - jt);
-        const bool more = jt + BLOCK < jend;
-        if (more && jt + BLOCK + t < jend) st = load_jbmx(a.s, jt + BLOCK + t);
-        const JBMX* tb = tile[buf];
-        for (int k0 = 0; k0 < cnt; k0 += kFold) {
-            float acc[IPL][6];
+    for (int j0 = jbeg; j0 < jend; j0 += kFold) {
+        float acc[IPL][6];
 #pragma unroll
-            for (int q = 0; q < IPL; ++q)
+        for (int q = 0; q < IPL; ++q)
 #pragma unroll
-                for (int c = 0; c < 6; ++c) acc[q][c] = 0.f;
-            if (k0 + kFold <= cnt) {
+            for (int c = 0; c < 6; ++c) acc[q][c] = 0.f;
+        if (j0 + kFold <= jend) {
 #pragma unroll UNROLL
-                for (int k = 0; k < kFold; ++k) {
-                    const JBMX b = tb[k0 + k];
+            for (int j = j0; j < j0 + kFold; ++j) {
+                const JBMX b = rec[j];
 #pragma unroll
-                    for (int q = 0; q < IPL; ++q)
-                        pair_mx<MASK>(b, jt + k0 + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q],
-                                      vzi[q], eps2, acc[q]);
-                }
-            } else {
-                for (int k = k0; k < cnt; ++k) {
-                    const JBMX b = tb[k];
-#pragma unroll
-                    for (int q = 0; q < IPL; ++q)
-                        pair_mx<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
-                                      eps2, acc[q]);
-                }
+                for (int q = 0; q < IPL; ++q)
+                    pair_mx<MASK>(b, j, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q], eps2,
+                                  acc[q]);
             }
+        } else {
+            for (int j = j0; j < jend; ++j) {
+                const JBMX b = rec[j];
 #pragma unroll
-            for (int q = 0; q < IPL; ++q)
-#pragma unroll
-                for (int c = 0; c < 6; ++c) two_sum_fold(hi[q][c], lo[q][c], acc[q][c]);
+                for (int q = 0; q < IPL; ++q)
+                    pair_mx<MASK>(b, j, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q], eps2,
+                                  acc[q]);
+            }
         }
-        if (more) tile[buf ^ 1][t] = st;
-        __syncthreads();
-        buf ^= 1;
+#pragma unroll
+        for (int q = 0; q < IPL; ++q)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) two_sum_fold(hi[q][c], lo[q][c], acc[q][c]);
     }
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
@@ -456,7 +457,7 @@ __global__ __launch_bounds__(kOBlock) void k_check_finite(int n, Forces f, int* 
 inline int oblocks(int n) { return (n + kOBlock - 1) / kOBlock; }
 
 template <int BLOCK, int IPL>
-hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, hipStream_t st)
+hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, const JBMX* rec, hipStream_t st)
 {
     const dim3 grid((a.n + BLOCK * IPL - 1) / (BLOCK * IPL), jsplit);
     if (precision == 0) {
@@ -466,18 +467,29 @@ hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, 
             k_force_f64<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a);
     } else {
         if (mask)
-            k_force_mixed<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a);
+            k_force_mixed<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a, rec);
         else
-            k_force_mixed<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a);
+            k_force_mixed<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a, rec);
     }
     return hipGetLastError();
 }
 
+
 }  // namespace
+
+size_t source_record_bytes(int precision, int n) { return precision == 0 ? 0 : sizeof(JBMX) * (size_t)n; }
+
+hipError_t pack_source_records(int precision, const State& s, int n, void* rec, hipStream_t st)
+{
+    if (precision == 0) return hipSuccess;
+    if (!rec) return hipErrorInvalidValue;
+    k_pack_mixed<<<(n + kOBlock - 1) / kOBlock, kOBlock, 0, st>>>(s, n, static_cast<JBMX*>(rec));
+    return hipGetLastError();
+}
 
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out, double* partial,
-                        hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int nj)
+                        void* rec, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int nj)
 {
     PairArgs a;
     a.s = s;
@@ -488,7 +500,7 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
     a.diag = 0;
     if (shape.sym_chunk > 0) {
         if (nj > 0) return hipErrorInvalidValue;
-        return launch_force_sym(precision, s, n, eps2, shape.sym_chunk, out, st, ev0, ev1);
+        return launch_force_sym(precision, s, n, eps2, shape.sym_chunk, out, rec, st, ev0, ev1);
     }
     if (shape.jsplit > 1) {
         // partial plane for (component c, split k) starts at (c*S + k)*n
@@ -499,16 +511,18 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
         a.split_stride = 0;
     }
     const bool mask = (eps2 == 0.0);
+    const JBMX* r = static_cast<const JBMX*>(rec);
     hipError_t e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
+    if ((e = pack_source_records(precision, s, a.nj, rec, st)) != hipSuccess) return e;
     const int key = shape.block * 10 + shape.ipl;
     switch (key) {
-    case 641: e = launch_pair<64, 1>(precision, mask, a, shape.jsplit, st); break;
-    case 642: e = launch_pair<64, 2>(precision, mask, a, shape.jsplit, st); break;
-    case 1281: e = launch_pair<128, 1>(precision, mask, a, shape.jsplit, st); break;
-    case 1282: e = launch_pair<128, 2>(precision, mask, a, shape.jsplit, st); break;
-    case 2561: e = launch_pair<256, 1>(precision, mask, a, shape.jsplit, st); break;
-    case 2562: e = launch_pair<256, 2>(precision, mask, a, shape.jsplit, st); break;
+    case 641: e = launch_pair<64, 1>(precision, mask, a, shape.jsplit, r, st); break;
+    case 642: e = launch_pair<64, 2>(precision, mask, a, shape.jsplit, r, st); break;
+    case 1281: e = launch_pair<128, 1>(precision, mask, a, shape.jsplit, r, st); break;
+    case 1282: e = launch_pair<128, 2>(precision, mask, a, shape.jsplit, r, st); break;
+    case 2561: e = launch_pair<256, 1>(precision, mask, a, shape.jsplit, r, st); break;
+    case 2562: e = launch_pair<256, 2>(precision, mask, a, shape.jsplit, r, st); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -521,10 +535,12 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
 }
 
 hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
-                                  const Forces& out, hipStream_t st)
+                                  const Forces& out, void* rec, hipStream_t st)
 {
     constexpr int BLOCK = 256;
     if (B <= 0 || B % BLOCK != 0) return hipErrorInvalidValue;
+    hipError_t e = pack_source_records(precision, s, n, rec, st);
+    if (e != hipSuccess) return e;
     PairArgs a;
     a.s = s;
     a.n = n;
@@ -534,7 +550,7 @@ hipError_t launch_force_blockdiag(int precision, const State& s, int n, double e
     a.out = out;
     a.split_stride = 0;
     a.diag = B;
-    return launch_pair<BLOCK, 1>(precision, eps2 == 0.0, a, 1, st);
+    return launch_pair<BLOCK, 1>(precision, eps2 == 0.0, a, 1, static_cast<const JBMX*>(rec), st);
 }
 
 hipError_t launch_predict(int n, const double* dt, double* x, double* y, double* z,
