@@ -17,8 +17,10 @@ struct __align__(16) JB64 {
     double x, y, z, m, vx, vy, vz, pad;
 };
 
-// Mixed-precision source body in LDS: FP32 velocity and mass first, FP64
-// position after, 48 bytes -> two 16-byte and one 8-byte broadcast.
+// Mixed-precision source body: FP32 velocity and mass first, FP64 position
+// after, 48 bytes.  Held in LDS tiles (ensemble, symmetric kernels: two
+// 16-byte and one 8-byte read) or packed in HBM for the single-system
+// directed kernel, which reads it with scalar loads.
 struct __align__(16) JBMX {
     float vx, vy, vz, m;
     double x, y, z, pad;
