@@ -147,6 +147,63 @@ def test_mixed_vs_oracle(bemx, oracle, nb):
     assert np.median(ej) <= 1e-5
 
 
+def check_mixed(f, ref):
+    ea = body_relerr(f[:3], ref[:3])
+    ej = body_relerr(f[3:], ref[3:])
+    assert np.median(ea) <= 1e-6 and ea.max() <= 1e-5, f"acc median {np.median(ea):.2e} max {ea.max():.2e}"
+    assert np.median(ej) <= 1e-5, f"jerk median {np.median(ej):.2e}"
+    return np.median(ea), ea.max()
+
+
+# the mixed directed kernel reads its sources from packed records at
+# wave-uniform indices: ragged ends of fold runs, splits and targets
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 65, 257, 1000, 1023, 1025, 4097])
+def test_mixed_ragged_sizes(bemx, oracle, nb, n):
+    _banner(f"mixed ragged n={n}")
+    s = nb.plummer(n, 2000 + n)
+    bemx.configure(eps2=1e-4, block=0, jsplit=0, ipl=0)
+    f = bemx.force(s)
+    if n == 1:
+        assert np.all(f == 0.0)
+    else:
+        check_mixed(f, oracle.force(s, 1e-4))
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("jsplit", [1, 2, 7, 0])
+@pytest.mark.parametrize("ipl", [1, 2])
+def test_mixed_launch_shapes_agree(bemx, oracle, nb, block, jsplit, ipl):
+    _banner(f"mixed shape block={block} jsplit={jsplit} ipl={ipl}")
+    s = nb.plummer(3000, 77)
+    bemx.configure(eps2=1e-3, block=block, jsplit=jsplit, ipl=ipl)
+    try:
+        check_mixed(bemx.force(s), oracle.force(s, 1e-3))
+    finally:
+        bemx.configure(block=0, jsplit=0, ipl=0)
+
+
+def test_mixed_zero_softening(bemx, oracle, nb):
+    """eps2 = 0: the self pair is dropped in the mixed kernel too."""
+    s = nb.plummer(777, 8)
+    bemx.configure(eps2=0.0)
+    f = bemx.force(s)
+    assert np.isfinite(f).all()
+    check_mixed(f, oracle.force(s, 0.0))
+
+
+def test_mixed_autotune_truncated_trials(bemx, oracle, nb):
+    """n > 65536: the autotuner's first stage times a truncated source range
+    (the packed records cover only those sources), the second all of them."""
+    n = 70000
+    s = nb.plummer(n, 81)
+    bemx.configure(eps2=1e-4, block=0, jsplit=0, ipl=0)
+    t = bemx.autotune(n)
+    _banner(f"mixed autotune n={n}: {t}")
+    f = bemx.force(s)
+    idx = np.sort(np.random.default_rng(5).choice(n, 192, replace=False))
+    check_mixed(f[:, idx], oracle.force_sample(s, 1e-4, idx, nthreads=16))
+
+
 def test_mixed_vs_reference_f32_fixture(bemx):
     """The reference's _F32 build is FP32 arithmetic with double sums; the
     mixed kernel must sit within FP32 rounding of it."""
