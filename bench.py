@@ -95,7 +95,7 @@ def measured_traffic():
 
 def kname(prec, res):
     """Pair-kernel instance of a run: k_force_{f64,mixed}<block,targets per lane>
-    or the symmetric kernel k_force_mixed_sym<targets per lane, waves>."""
+    or the symmetric kernel k_force_sym<SymF64|SymMixed,targets per lane,waves>."""
     sh = res["shape"]
     if sh.get("pairing") == "symmetric":
         return f"k_force_sym<{'SymF64' if prec == 'fp64' else 'SymMixed'},{sh['ipl']},{sh['block'] // 64}>"
