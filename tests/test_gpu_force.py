@@ -276,6 +276,33 @@ def test_force_device_pointers(be64, oracle, nb):
     check64(out, oracle.force(s, 1e-4))
 
 
+def test_force_device_pointers_mixed(bemx, oracle, nb):
+    """nbh_force_device in mixed precision: the context packs the caller's
+    source arrays into its own record buffer, which grows with n."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    for n in (1500, 4100, 700):
+        s = np.ascontiguousarray(nb.plummer(n, 30 + n))
+        out = np.zeros((6, n))
+        d_in, d_out = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(d_in), C.c_size_t(s.nbytes)) == 0
+        assert hip.hipMalloc(C.byref(d_out), C.c_size_t(out.nbytes)) == 0
+        try:
+            assert hip.hipMemcpy(d_in, s.ctypes.data_as(C.c_void_p), C.c_size_t(s.nbytes), 1) == 0
+            bemx.configure(eps2=1e-4, block=0, jsplit=0, ipl=0)
+            col = lambda base, k: C.c_void_p(base.value + 8 * n * k)  # noqa: E731
+            rc = nb.lib().nbh_force_device(bemx._h, n, *[col(d_in, k) for k in range(7)],
+                                           *[col(d_out, k) for k in range(6)], None)
+            assert rc == nb.OK
+            bemx.timing()  # drains the context stream
+            assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), d_out, C.c_size_t(out.nbytes), 2) == 0
+        finally:
+            hip.hipFree(d_in)
+            hip.hipFree(d_out)
+        check_mixed(out, oracle.force(s, 1e-4))
+        assert np.array_equal(out, bemx.force(s))  # same kernel through the host-array call
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_force_device_pointers_symmetric(oracle, nb, precision):
     """nbh_force_device with symmetric pairing: the atomic totals land in the
