@@ -40,9 +40,9 @@ struct Shape {
 // and the split-reduction kernel is launched after the pair kernel.  ev0/ev1
 // (may be null) are recorded around the pair kernel only.
 // rec: device buffer of source_record_bytes(precision, n) bytes for the
-// mixed kernels' packed source records (unused, may be null, in FP64).
+// directed kernels' packed source records.
 size_t source_record_bytes(int precision, int n);
-// Packs sources 0..n-1 of s into rec (mixed; nothing to do in FP64).
+// Packs sources 0..n-1 of s into rec.
 hipError_t pack_source_records(int precision, const State& s, int n, void* rec, hipStream_t st);
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out,

@@ -5,18 +5,13 @@
 // kernels/nb_compute.cpp:180-582):
 //   * one or two target (i) bodies per lane, BLOCK lanes per workgroup;
 //   * every lane of a wave takes the same source (j) body at the same time,
-//     so the source record is wave-uniform:
-//       - FP64: sources streamed from HBM in tiles of BLOCK bodies, each lane
-//         loading one body with coalesced SoA loads into an LDS tile
-//         (array-of-structs, 16-byte aligned; two tiles alternate so the
-//         loads of tile t+1 overlap the arithmetic on tile t), and read back
-//         at a wave-uniform address, a broadcast to the 64 lanes;
-//       - mixed: sources packed once per evaluation into 48-byte records
-//         (FP64 position, FP32 velocity and mass) and read with scalar loads
-//         into SGPRs, which the VALU instructions take as operands: no LDS,
-//         no barriers, and the registers of a record are scalar, so twice as
-//         many waves stay resident (FP32 reaches its rate only by pairing
-//         instructions of two waves);
+//     so the source record is wave-uniform.  Sources are packed once per
+//     evaluation into array-of-structs records (FP64: 64 bytes; mixed: 48,
+//     FP64 position with FP32 velocity and mass) and read at wave-uniform
+//     indices, i.e. with scalar loads into SGPRs, which the VALU
+//     instructions take as operands: no LDS, no barriers, and a record's
+//     registers are scalar, so more waves stay resident (FP32 reaches its
+//     rate only by pairing instructions of two waves);
 //   * grid.y splits the j-range when n is too small to fill the CUs; the
 //     partial sums are added by a second kernel in split order, so results
 //     are deterministic run to run;
@@ -62,14 +57,28 @@ __device__ __forceinline__ void source_range(const PairArgs& a, int& jbeg, int& 
     }
 }
 
-// BLOCK lanes per workgroup, IPL targets per lane (lane t holds targets
-// base + t + q*BLOCK, q < IPL), so each broadcast source record read from
-// LDS feeds IPL pair evaluations.
-template <int BLOCK, int IPL, bool MASK>
-__global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
+// Source records of both precisions, packed from the SoA state before each
+// evaluation.
+__global__ __launch_bounds__(kOBlock) void k_pack_f64(State s, int n, JB64* __restrict__ rec)
 {
-    constexpr int UNROLL = IPL == 1 ? 4 : 2;
-    __shared__ JB64 tile[2][BLOCK];
+    const int j = blockIdx.x * kOBlock + threadIdx.x;
+    if (j < n) rec[j] = load_jb64(s, j);
+}
+
+__global__ __launch_bounds__(kOBlock) void k_pack_mixed(State s, int n, JBMX* __restrict__ rec)
+{
+    const int j = blockIdx.x * kOBlock + threadIdx.x;
+    if (j < n) rec[j] = load_jbmx(s, j);
+}
+
+// BLOCK lanes per workgroup, IPL targets per lane (lane t holds targets
+// base + t + q*BLOCK, q < IPL), so each source record (from `rec`, at a
+// wave-uniform index: scalar loads) feeds IPL pair evaluations.  Each target
+// sums its sources in order, whatever BLOCK and IPL.
+template <int BLOCK, int IPL, bool MASK>
+__global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a, const JB64* __restrict__ rec)
+{
+    constexpr int UNROLL = 4;
     const int t = threadIdx.x;
     const int base = blockIdx.x * BLOCK * IPL + t;
     const double eps2 = a.eps2;
@@ -87,37 +96,12 @@ __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
 
     int jbeg, jend;
     source_range<BLOCK, IPL>(a, jbeg, jend);
-    JB64 st{};
-    if (jbeg + t < jend) st = load_jb64(a.s, jbeg + t);
-    tile[0][t] = st;
-    __syncthreads();
-    int buf = 0;
-    for (int jt = jbeg; jt < jend; jt += BLOCK) {
-        const int cnt = min(BLOCK, jend - jt);
-        const bool more = jt + BLOCK < jend;
-        if (more && jt + BLOCK + t < jend) st = load_jb64(a.s, jt + BLOCK + t);
-        const JB64* tb = tile[buf];
-        if (cnt == BLOCK) {
 #pragma unroll UNROLL
-            for (int k = 0; k < BLOCK; ++k) {
-                const JB64 b = tb[k];
+    for (int j = jbeg; j < jend; ++j) {
+        const JB64 b = rec[j];
 #pragma unroll
-                for (int q = 0; q < IPL; ++q)
-                    pair_f64<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
-                                   eps2, acc[q]);
-            }
-        } else {
-            for (int k = 0; k < cnt; ++k) {
-                const JB64 b = tb[k];
-#pragma unroll
-                for (int q = 0; q < IPL; ++q)
-                    pair_f64<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
-                                   eps2, acc[q]);
-            }
-        }
-        if (more) tile[buf ^ 1][t] = st;
-        __syncthreads();
-        buf ^= 1;
+        for (int q = 0; q < IPL; ++q)
+            pair_f64<MASK>(b, j, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q], eps2, acc[q]);
     }
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
@@ -128,13 +112,6 @@ __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a)
             for (int c = 0; c < 6; ++c) a.out.c[c][o] = acc[q][c];
         }
     }
-}
-
-// Mixed source records, packed from the SoA state before each evaluation.
-__global__ __launch_bounds__(kOBlock) void k_pack_mixed(State s, int n, JBMX* __restrict__ rec)
-{
-    const int j = blockIdx.x * kOBlock + threadIdx.x;
-    if (j < n) rec[j] = load_jbmx(s, j);
 }
 
 // Sources from `rec` (k_pack_mixed) at wave-uniform indices: scalar loads.
@@ -457,19 +434,21 @@ __global__ __launch_bounds__(kOBlock) void k_check_finite(int n, Forces f, int* 
 inline int oblocks(int n) { return (n + kOBlock - 1) / kOBlock; }
 
 template <int BLOCK, int IPL>
-hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, const JBMX* rec, hipStream_t st)
+hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, const void* rec, hipStream_t st)
 {
     const dim3 grid((a.n + BLOCK * IPL - 1) / (BLOCK * IPL), jsplit);
     if (precision == 0) {
+        const JB64* r = static_cast<const JB64*>(rec);
         if (mask)
-            k_force_f64<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a);
+            k_force_f64<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a, r);
         else
-            k_force_f64<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a);
+            k_force_f64<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a, r);
     } else {
+        const JBMX* r = static_cast<const JBMX*>(rec);
         if (mask)
-            k_force_mixed<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a, rec);
+            k_force_mixed<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a, r);
         else
-            k_force_mixed<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a, rec);
+            k_force_mixed<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a, r);
     }
     return hipGetLastError();
 }
@@ -477,13 +456,18 @@ hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, 
 
 }  // namespace
 
-size_t source_record_bytes(int precision, int n) { return precision == 0 ? 0 : sizeof(JBMX) * (size_t)n; }
+size_t source_record_bytes(int precision, int n)
+{
+    return (precision == 0 ? sizeof(JB64) : sizeof(JBMX)) * (size_t)n;
+}
 
 hipError_t pack_source_records(int precision, const State& s, int n, void* rec, hipStream_t st)
 {
-    if (precision == 0) return hipSuccess;
     if (!rec) return hipErrorInvalidValue;
-    k_pack_mixed<<<(n + kOBlock - 1) / kOBlock, kOBlock, 0, st>>>(s, n, static_cast<JBMX*>(rec));
+    if (precision == 0)
+        k_pack_f64<<<(n + kOBlock - 1) / kOBlock, kOBlock, 0, st>>>(s, n, static_cast<JB64*>(rec));
+    else
+        k_pack_mixed<<<(n + kOBlock - 1) / kOBlock, kOBlock, 0, st>>>(s, n, static_cast<JBMX*>(rec));
     return hipGetLastError();
 }
 
@@ -511,7 +495,7 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
         a.split_stride = 0;
     }
     const bool mask = (eps2 == 0.0);
-    const JBMX* r = static_cast<const JBMX*>(rec);
+    const void* r = rec;
     hipError_t e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
     if ((e = pack_source_records(precision, s, a.nj, rec, st)) != hipSuccess) return e;
@@ -550,7 +534,7 @@ hipError_t launch_force_blockdiag(int precision, const State& s, int n, double e
     a.out = out;
     a.split_stride = 0;
     a.diag = B;
-    return launch_pair<BLOCK, 1>(precision, eps2 == 0.0, a, 1, static_cast<const JBMX*>(rec), st);
+    return launch_pair<BLOCK, 1>(precision, eps2 == 0.0, a, 1, rec, st);
 }
 
 hipError_t launch_predict(int n, const double* dt, double* x, double* y, double* z,

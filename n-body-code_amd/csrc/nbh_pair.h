@@ -11,16 +11,16 @@
 
 namespace nbh {
 
-// FP64 source body in LDS: 64 bytes, read as three 16-byte and one 8-byte
-// broadcast (pad is never read).
+// FP64 source body: 64 bytes (pad is never read).  Held in LDS tiles by the
+// ensemble kernel (three 16-byte and one 8-byte broadcast read) and packed in
+// HBM for the single-system directed kernel, which reads it with scalar loads.
 struct __align__(16) JB64 {
     double x, y, z, m, vx, vy, vz, pad;
 };
 
 // Mixed-precision source body: FP32 velocity and mass first, FP64 position
 // after, 48 bytes.  Held in LDS tiles (ensemble, symmetric kernels: two
-// 16-byte and one 8-byte read) or packed in HBM for the single-system
-// directed kernel, which reads it with scalar loads.
+// 16-byte and one 8-byte read) or packed in HBM like JB64.
 struct __align__(16) JBMX {
     float vx, vy, vz, m;
     double x, y, z, pad;
