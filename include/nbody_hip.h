@@ -53,6 +53,19 @@ enum nbh_pairing {
     NBH_PAIR_SYMMETRIC = 1
 };
 
+/* How the directed kernels bring in their source bodies (the same sums
+ * either way, bit for bit).  LDS: tiles loaded by all 64 lanes at once and
+ * read back as broadcasts; best for small n, where few waves are resident to
+ * hide load latency.  SCALAR: sources packed once per evaluation into
+ * records in HBM and read with scalar loads into SGPRs, with no LDS and more
+ * waves resident; best for large n.  AUTO: the autotuner's choice for n's
+ * size bucket, else SCALAR from n >= 65536 (mixed) / 262144 (FP64). */
+enum nbh_sources {
+    NBH_SRC_AUTO = 0,
+    NBH_SRC_LDS = 1,
+    NBH_SRC_SCALAR = 2
+};
+
 typedef struct nbh_ctx nbh_ctx;
 
 typedef struct nbh_config {
@@ -60,15 +73,15 @@ typedef struct nbh_config {
     int precision;  /* enum nbh_precision */
     double eps2;    /* Plummer softening squared; the reference hard-codes
                        EPS2 = 1e-7 (include/kernel.h:13) */
-    int block;      /* i-bodies (= threads) per workgroup and, in FP64, the
-                       LDS j-tile length: 64, 128 or 256; 0 = pick by size
-                       (the mixed kernel reads sources through scalar loads
-                       and has no tile) */
+    int block;      /* i-bodies (= threads) per workgroup, and the j-tile
+                       length with LDS sources: 64, 128 or 256; 0 = pick by
+                       size */
     int jsplit;     /* j-range split across workgroups; 0 = pick by size */
     int ipl;        /* targets per lane, 1 or 2; 0 = pick by size */
     int pairing;    /* enum nbh_pairing; 0 = NBH_PAIR_DIRECTED.  block,
-                       jsplit and ipl shape the directed kernels; the
-                       symmetric kernels have a fixed shape */
+                       jsplit, ipl and sources shape the directed kernels;
+                       the symmetric kernels have a fixed shape */
+    int sources;    /* enum nbh_sources; 0 = NBH_SRC_AUTO */
 } nbh_config;
 
 /* Description of the last failure on this thread ("" if none). */
@@ -176,15 +189,20 @@ int nbh_launch_shape(nbh_ctx *ctx, int n, int *block, int *jsplit,
 int nbh_launch_ipl(nbh_ctx *ctx, int n, int *ipl);
 /* Pairing (enum nbh_pairing) that a force evaluation of n bodies uses. */
 int nbh_launch_pairing(nbh_ctx *ctx, int n, int *pairing);
+/* Source path of that shape: NBH_SRC_LDS or NBH_SRC_SCALAR (directed), 0
+ * for the symmetric kernel. */
+int nbh_launch_sources(nbh_ctx *ctx, int n, int *sources);
 
 /* Autotuner of the single-system force: times block {64,128,256} x targets
- * per lane {1,2} x j-split {1..32} for n bodies on this context's device and
+ * per lane {1,2} x j-split {1..32} x sources {LDS, SCALAR} (only the
+ * config's, when it sets one) for n bodies on this context's device and
  * precision (a Plummer sphere, on a source range of max(min(n, 65536), n/16)
  * bodies so that a trial is short; when that range is truncated, the three
  * fastest shapes, and the fastest one with half and twice its j-split, are
  * timed again on all n sources) and keeps the fastest
  * for n's size bucket (floor(log2 n)) and precision.  Later evaluations in
- * that bucket use it unless the config forces block/jsplit/ipl.  Directed
+ * that bucket use it unless the config forces block/jsplit/ipl (and its
+ * source path unless the config sets one).  Directed
  * pairing only: the symmetric kernel has a fixed shape.  Outputs may be
  * NULL; ms is the winner's last timed launch. */
 int nbh_autotune(nbh_ctx *ctx, int n, int *block, int *ipl, int *jsplit,

@@ -262,7 +262,6 @@ struct nbh_adjoint {
     DevBuf g;        // [7][n]
     DevBuf mbar;     // [n]
     DevBuf partial;  // [splits][7][n] (also the force split partials)
-    DevBuf rec;      // the force kernel's packed source records
     DevBuf dtbuf;    // [1]
 };
 
@@ -284,7 +283,7 @@ int adj_force(nbh_adjoint* a, double* st, double* f)
     sh.jchunk = per * 64;
     sh.jsplit = (n + sh.jchunk - 1) / sh.jchunk;
     hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(st, n), n, c->cfg.eps2, sh, nbh::forces_of(f, n),
-                                     a->partial.d(), a->rec.p, c->stream, nullptr, nullptr);
+                                     a->partial.d(), nullptr, c->stream, nullptr, nullptr);  // LDS tiles
     if (e != hipSuccess) return fail(NBH_ERR_HIP, "adjoint force launch failed: %s", hipGetErrorString(e));
     return NBH_OK;
 }
@@ -359,8 +358,7 @@ int nbh_adj_create(nbh_ctx* c, int n, int every, nbh_adjoint** out)
         a->f1.reserve(sz(n, 6)) != hipSuccess || a->pred.reserve(sz(n, 7)) != hipSuccess ||
         a->cot.reserve(sz(n, 12)) != hipSuccess || a->cf.reserve(sz(n, 6)) != hipSuccess ||
         a->g.reserve(sz(n, 7)) != hipSuccess || a->mbar.reserve(sz(n, 1)) != hipSuccess ||
-        a->partial.reserve(sz(n, planes)) != hipSuccess || a->dtbuf.reserve(sizeof(double)) != hipSuccess ||
-        a->rec.reserve(nbh::source_record_bytes(NBH_FP64, n)) != hipSuccess) {
+        a->partial.reserve(sz(n, planes)) != hipSuccess || a->dtbuf.reserve(sizeof(double)) != hipSuccess) {
         nbh_adj_destroy(a);
         return fail(NBH_ERR_ALLOC, "adjoint device allocation failed (n=%d, every=%d)", n, every);
     }
@@ -374,7 +372,7 @@ int nbh_adj_destroy(nbh_adjoint* a)
     (void)hipSetDevice(a->ctx->cfg.device);
     (void)hipStreamSynchronize(a->ctx->stream);
     for (DevBuf* b : {&a->ckpt, &a->seg, &a->cur, &a->f1, &a->pred, &a->cot, &a->cf, &a->g, &a->mbar, &a->partial,
-                      &a->dtbuf, &a->rec})
+                      &a->dtbuf})
         b->release();
     delete a;
     return NBH_OK;

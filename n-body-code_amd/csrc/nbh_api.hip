@@ -101,13 +101,21 @@ int size_bucket(int n)
     return b;
 }
 
-// Shape with the given block / targets-per-lane / requested split; the split
-// is clamped to the source tiles and rounded to whole tiles per split.
-nbh::Shape make_shape(int n, int block, int ipl, int split, int nj = 0)
+// Source path when neither the config nor the autotuner sets one: scalar
+// loads once enough waves are resident to hide their latency (measured: the
+// LDS kernels win below, e.g. 2.5x at n = 1024; the scalar-load ones above,
+// by 12% (mixed) and 2.6% (FP64) at n = 2^20).
+bool default_scalar(int precision, int n) { return n >= (precision == NBH_MIXED ? 65536 : 262144); }
+
+// Shape with the given block / targets-per-lane / requested split / source
+// path; the split is clamped to the source tiles and rounded to whole tiles
+// per split.
+nbh::Shape make_shape(int n, int block, int ipl, int split, bool scalar, int nj = 0)
 {
     nbh::Shape s{};
     s.block = block;
     s.ipl = ipl;
+    s.scalar = scalar ? 1 : 0;
     const int src = nj > 0 ? nj : n;
     const int jtiles = (src + block - 1) / block;
     split = std::max(1, std::min(split, jtiles));
@@ -129,10 +137,13 @@ nbh::Shape pick_shape(const nbh_ctx* c, int n)
         s.sym_chunk = nbh::sym_chunk(f.precision, n, c->cus);
         return s;
     }
+    const bool forced_src = f.sources != NBH_SRC_AUTO;
     if (f.block == 0 && f.jsplit == 0 && f.ipl == 0) {
         const auto& t = c->tuned[f.precision][size_bucket(n)];
-        if (t.block) return make_shape(n, t.block, t.ipl, t.jsplit);
+        if (t.block)
+            return make_shape(n, t.block, t.ipl, t.jsplit, forced_src ? f.sources == NBH_SRC_SCALAR : t.scalar != 0);
     }
+    const bool scalar = forced_src ? f.sources == NBH_SRC_SCALAR : default_scalar(f.precision, n);
     const int block = f.block ? f.block : (n >= 131072 ? 256 : (n >= 16384 ? 128 : 64));
     const int ipl = f.ipl ? f.ipl : 1;
     int split = f.jsplit;
@@ -143,7 +154,7 @@ nbh::Shape pick_shape(const nbh_ctx* c, int n)
         const int want = 128 * (c->cus > 0 ? c->cus : 256);
         split = std::min(64, (want + iblocks - 1) / iblocks);
     }
-    return make_shape(n, block, ipl, split);
+    return make_shape(n, block, ipl, split, scalar);
 }
 
 int check_cfg(const nbh_config* cfg)
@@ -159,6 +170,8 @@ int check_cfg(const nbh_config* cfg)
     if (cfg->ipl < 0 || cfg->ipl > 2) return fail(NBH_ERR_ARG, "ipl must be 0, 1 or 2, got %d", cfg->ipl);
     if (cfg->pairing != NBH_PAIR_DIRECTED && cfg->pairing != NBH_PAIR_SYMMETRIC)
         return fail(NBH_ERR_ARG, "pairing must be NBH_PAIR_DIRECTED or NBH_PAIR_SYMMETRIC, got %d", cfg->pairing);
+    if (cfg->sources != NBH_SRC_AUTO && cfg->sources != NBH_SRC_LDS && cfg->sources != NBH_SRC_SCALAR)
+        return fail(NBH_ERR_ARG, "sources must be NBH_SRC_AUTO, NBH_SRC_LDS or NBH_SRC_SCALAR, got %d", cfg->sources);
     return NBH_OK;
 }
 
@@ -180,7 +193,7 @@ int enqueue_force(nbh_ctx* c, const nbh::State& s, int n, const nbh::Forces& out
         NBH_HIP(c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n));
         part = c->partial.d();
     }
-    NBH_HIP(c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)));
+    if (sh.scalar) NBH_HIP(c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)));
     EventPair ep;
     const bool timed = c->profiling && allow_events;
     if (timed) {
@@ -202,7 +215,7 @@ extern "C" {
 
 const char* nbh_last_error(void) { return nbh::last_error(); }
 
-int nbh_version(void) { return 101; }
+int nbh_version(void) { return 102; }
 
 int nbh_device_info(int device, int* compute_units, int* wave_size, char* name, int name_len)
 {
@@ -304,6 +317,15 @@ int nbh_launch_shape(nbh_ctx* c, int n, int* block, int* jsplit, int* workgroups
     return NBH_OK;
 }
 
+int nbh_launch_sources(nbh_ctx* c, int n, int* sources)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    const nbh::Shape s = pick_shape(c, n);
+    if (sources) *sources = s.sym_chunk > 0 ? 0 : (s.scalar ? NBH_SRC_SCALAR : NBH_SRC_LDS);
+    return NBH_OK;
+}
+
 int nbh_launch_pairing(nbh_ctx* c, int n, int* pairing)
 {
     if (!c) return fail(NBH_ERR_ARG, "context is NULL");
@@ -342,7 +364,7 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
     NBH_HIP(hipEventCreate(&e1));
     struct Trial {
         double ms;
-        int block, ipl, split;
+        int block, ipl, split, scalar;
     };
     std::vector<Trial> trials;
     int rc = NBH_OK;
@@ -350,7 +372,7 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
     auto time_shape = [&](const nbh::Shape& sh, int reps, int trial_nj, double* out_ms) -> int {
         if (sh.jsplit > 1 && c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n) != hipSuccess)
             return fail(NBH_ERR_ALLOC, "autotune: partial-sum allocation failed");
-        if (c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)) != hipSuccess)
+        if (sh.scalar && c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)) != hipSuccess)
             return fail(NBH_ERR_ALLOC, "autotune: source-record allocation failed");
         for (int rep = 0; rep < reps; ++rep) {
             hipError_t e = hipEventRecord(e0, c->stream);
@@ -366,25 +388,30 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
         }
         return (int)NBH_OK;
     };
-    // stage 1: every shape on the truncated source range
-    for (int block : {64, 128, 256}) {
-        if (n >= 65536 && block == 64) continue;
-        for (int ipl : {1, 2}) {
-            int last = -1;
-            for (int split : {1, 2, 4, 8, 16, 32}) {
-                const nbh::Shape sh = make_shape(n, block, ipl, split, nj);
-                if (sh.jsplit == last) continue;
-                last = sh.jsplit;
-                double ms = 0.0;
-                if ((rc = time_shape(sh, 2, nj, &ms)) != NBH_OK) break;  // the first launch warms up
-                trials.push_back({ms, block, ipl, split});
+    // stage 1: every shape on the truncated source range, with either source
+    // path unless the config sets one
+    const int src_lo = c->cfg.sources == NBH_SRC_SCALAR ? 1 : 0;
+    const int src_hi = c->cfg.sources == NBH_SRC_LDS ? 0 : 1;
+    for (int scalar = src_lo; scalar <= src_hi && rc == NBH_OK; ++scalar) {
+        for (int block : {64, 128, 256}) {
+            if (n >= 65536 && block == 64) continue;
+            for (int ipl : {1, 2}) {
+                int last = -1;
+                for (int split : {1, 2, 4, 8, 16, 32}) {
+                    const nbh::Shape sh = make_shape(n, block, ipl, split, scalar != 0, nj);
+                    if (sh.jsplit == last) continue;
+                    last = sh.jsplit;
+                    double ms = 0.0;
+                    if ((rc = time_shape(sh, 2, nj, &ms)) != NBH_OK) break;  // the first launch warms up
+                    trials.push_back({ms, block, ipl, split, scalar});
+                }
+                if (rc) break;
             }
             if (rc) break;
         }
-        if (rc) break;
     }
     double best = 1e300;
-    int bb = 0, bi = 0, bs = 0;
+    int bb = 0, bi = 0, bs = 0, bsc = 0;
     if (rc == NBH_OK) {
         std::sort(trials.begin(), trials.end(), [](const Trial& a, const Trial& b) { return a.ms < b.ms; });
         // stage 2 (truncated trials only): the three fastest again on all n
@@ -398,18 +425,22 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
                 if (sp < 1 || sp > 32) continue;
                 bool have = false;
                 for (size_t k = 0; k < top; ++k)
-                    have |= trials[k].block == b0.block && trials[k].ipl == b0.ipl && trials[k].split == sp;
-                if (!have) trials.insert(trials.begin() + top++, Trial{0.0, b0.block, b0.ipl, sp});
+                    have |= trials[k].block == b0.block && trials[k].ipl == b0.ipl && trials[k].split == sp &&
+                            trials[k].scalar == b0.scalar;
+                if (!have) trials.insert(trials.begin() + top++, Trial{0.0, b0.block, b0.ipl, sp, b0.scalar});
             }
         }
         for (size_t k = 0; k < top && rc == NBH_OK; ++k) {
             double ms = trials[k].ms;
-            if (full) rc = time_shape(make_shape(n, trials[k].block, trials[k].ipl, trials[k].split), 1, 0, &ms);
+            if (full)
+                rc = time_shape(make_shape(n, trials[k].block, trials[k].ipl, trials[k].split, trials[k].scalar != 0),
+                                1, 0, &ms);
             if (rc == NBH_OK && ms < best) {
                 best = ms;
                 bb = trials[k].block;
                 bi = trials[k].ipl;
                 bs = trials[k].split;
+                bsc = trials[k].scalar;
             }
         }
     }
@@ -420,6 +451,7 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
     t.block = bb;
     t.ipl = bi;
     t.jsplit = bs;
+    t.scalar = bsc;
     drop_graph(c);
     if (block_out) *block_out = bb;
     if (ipl_out) *ipl_out = bi;
@@ -460,7 +492,7 @@ int nbh_force(nbh_ctx* c, int n, const double* x, const double* y, const double*
             return fail(NBH_ERR_ALLOC, "device allocation of split partials failed");
         part = c->partial.d();
     }
-    if (c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)) != hipSuccess)
+    if (sh.scalar && c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)) != hipSuccess)
         return fail(NBH_ERR_ALLOC, "device allocation of source records failed");
     NBH_HIP(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
     hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, f, part, c->rec.p, c->stream,

@@ -81,9 +81,9 @@ struct nbh_ctx {
     int g_n = -1;
     bool g_adaptive = false;
 
-    // autotuned launch shapes, [precision][floor(log2 n)]: block, ipl, jsplit
+    // autotuned launch shapes, [precision][floor(log2 n)]: block, ipl, jsplit, source path
     struct Tuned {
-        int block = 0, ipl = 0, jsplit = 0;
+        int block = 0, ipl = 0, jsplit = 0, scalar = 0;
     } tuned[2][32];
 
     // profiling

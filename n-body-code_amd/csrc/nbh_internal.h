@@ -27,11 +27,12 @@ struct Forces {
 
 // One force evaluation's launch shape.
 struct Shape {
-    int block;   // lanes per workgroup = j-tile length (64, 128 or 256)
+    int block;   // lanes per workgroup (= the j-tile length of the LDS kernels): 64, 128 or 256
     int ipl;     // targets per lane (1 or 2): block*ipl targets per workgroup
     int jsplit;  // workgroups along j; partial sums reduced in a second pass
     int jchunk;  // j-bodies per split (multiple of block)
     int sym_chunk;  // > 0: symmetric kernel, block pairs per workgroup
+    int scalar;  // directed kernels: 1 = sources through scalar loads from packed records, 0 = LDS tiles
 };
 
 // Pair-kernel launch: writes forces for all n targets from all n sources
@@ -39,8 +40,8 @@ struct Shape {
 // launches).  When shape.jsplit > 1, `partial` must hold 6*jsplit*n doubles
 // and the split-reduction kernel is launched after the pair kernel.  ev0/ev1
 // (may be null) are recorded around the pair kernel only.
-// rec: device buffer of source_record_bytes(precision, n) bytes for the
-// directed kernels' packed source records.
+// rec: with shape.scalar, a device buffer of source_record_bytes(precision,
+// n) bytes for the packed source records (unused, may be null, otherwise).
 size_t source_record_bytes(int precision, int n);
 // Packs sources 0..n-1 of s into rec.
 hipError_t pack_source_records(int precision, const State& s, int n, void* rec, hipStream_t st);
@@ -53,7 +54,7 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
 // sources of its own block [B*(i/B), B*(i/B)+B), self-pair excluded; the
 // outputs are overwritten (the last block may be partial).  B % 256 == 0.
 hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
-                                  const Forces& out, void* rec, hipStream_t st);
+                                  const Forces& out, hipStream_t st);
 
 // Symmetric pair kernels (nbh_sym.hip): block-diagonal pass, then every
 // off-diagonal block pair once with the reaction added to the source side.
@@ -71,7 +72,7 @@ int sym_list_max(int precision, int n);  // longest source list of a full block
 int sym_chunk(int precision, int n, int cus);
 long sym_workgroups(int precision, int n, int chunk);
 hipError_t launch_force_sym(int precision, const State& s, int n, double eps2,
-                            int chunk, const Forces& out, void* rec, hipStream_t st,
+                            int chunk, const Forces& out, hipStream_t st,
                             hipEvent_t ev0, hipEvent_t ev1);
 
 // The ensemble's symmetric kernel cuts systems into blocks of its own size.

@@ -5,13 +5,20 @@
 // kernels/nb_compute.cpp:180-582):
 //   * one or two target (i) bodies per lane, BLOCK lanes per workgroup;
 //   * every lane of a wave takes the same source (j) body at the same time,
-//     so the source record is wave-uniform.  Sources are packed once per
-//     evaluation into array-of-structs records (FP64: 64 bytes; mixed: 48,
-//     FP64 position with FP32 velocity and mass) and read at wave-uniform
-//     indices, i.e. with scalar loads into SGPRs, which the VALU
-//     instructions take as operands: no LDS, no barriers, and a record's
-//     registers are scalar, so more waves stay resident (FP32 reaches its
-//     rate only by pairing instructions of two waves);
+//     so the source record (FP64: 64 bytes; mixed: 48, FP64 position with
+//     FP32 velocity and mass) is wave-uniform.  Two ways to bring it in:
+//       - LDS tiles (k_force_*_lds): each lane loads one body of a tile of
+//         BLOCK with coalesced SoA loads into LDS (two tiles alternate so
+//         the loads of tile t+1 overlap the arithmetic on tile t), read back
+//         as broadcasts.  Best for small n: 64 lanes fetch 64 sources at
+//         once, which hides the latency with few waves resident;
+//       - scalar loads (k_force_*): sources packed once per evaluation into
+//         records in HBM (k_pack_*) and read at wave-uniform indices into
+//         SGPRs, which the VALU instructions take as operands.  No LDS, no
+//         barriers, and a record's registers are scalar, so more waves stay
+//         resident; FP32 reaches its rate only by pairing instructions of
+//         two waves.  Best for large n;
+//     both sum each target's sources in the same order (bitwise the same);
 //   * grid.y splits the j-range when n is too small to fill the CUs; the
 //     partial sums are added by a second kernel in split order, so results
 //     are deterministic run to run;
@@ -54,6 +61,149 @@ __device__ __forceinline__ void source_range(const PairArgs& a, int& jbeg, int& 
     } else {
         jbeg = blockIdx.y * a.jchunk;
         jend = min(a.nj, jbeg + a.jchunk);
+    }
+}
+
+template <int BLOCK, int IPL, bool MASK>
+__global__ __launch_bounds__(BLOCK) void k_force_f64_lds(PairArgs a)
+{
+    constexpr int UNROLL = IPL == 1 ? 4 : 2;
+    __shared__ JB64 tile[2][BLOCK];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * BLOCK * IPL + t;
+    const double eps2 = a.eps2;
+    double xi[IPL], yi[IPL], zi[IPL], vxi[IPL], vyi[IPL], vzi[IPL];
+    double acc[IPL][6];
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int i = base + q * BLOCK;
+        const int ic = i < a.n ? i : a.n - 1;
+        xi[q] = a.s.x[ic]; yi[q] = a.s.y[ic]; zi[q] = a.s.z[ic];
+        vxi[q] = a.s.vx[ic]; vyi[q] = a.s.vy[ic]; vzi[q] = a.s.vz[ic];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[q][c] = 0.0;
+    }
+
+    int jbeg, jend;
+    source_range<BLOCK, IPL>(a, jbeg, jend);
+    JB64 st{};
+    if (jbeg + t < jend) st = load_jb64(a.s, jbeg + t);
+    tile[0][t] = st;
+    __syncthreads();
+    int buf = 0;
+    for (int jt = jbeg; jt < jend; jt += BLOCK) {
+        const int cnt = min(BLOCK, jend - jt);
+        const bool more = jt + BLOCK < jend;
+        if (more && jt + BLOCK + t < jend) st = load_jb64(a.s, jt + BLOCK + t);
+        const JB64* tb = tile[buf];
+        if (cnt == BLOCK) {
+#pragma unroll UNROLL
+            for (int k = 0; k < BLOCK; ++k) {
+                const JB64 b = tb[k];
+#pragma unroll
+                for (int q = 0; q < IPL; ++q)
+                    pair_f64<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
+                                   eps2, acc[q]);
+            }
+        } else {
+            for (int k = 0; k < cnt; ++k) {
+                const JB64 b = tb[k];
+#pragma unroll
+                for (int q = 0; q < IPL; ++q)
+                    pair_f64<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
+                                   eps2, acc[q]);
+            }
+        }
+        if (more) tile[buf ^ 1][t] = st;
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int i = base + q * BLOCK;
+        if (i < a.n) {
+            const size_t o = (size_t)blockIdx.y * a.split_stride + i;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a.out.c[c][o] = acc[q][c];
+        }
+    }
+}
+
+template <int BLOCK, int IPL, bool MASK>
+__global__ __launch_bounds__(BLOCK) void k_force_mixed_lds(PairArgs a)
+{
+    static_assert(BLOCK % kFold == 0, "tile must hold whole fold runs");
+    constexpr int UNROLL = IPL == 1 ? 4 : 2;
+    __shared__ JBMX tile[2][BLOCK];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * BLOCK * IPL + t;
+    const float eps2 = (float)a.eps2;
+    double xi[IPL], yi[IPL], zi[IPL];
+    float vxi[IPL], vyi[IPL], vzi[IPL];
+    float hi[IPL][6], lo[IPL][6];
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int i = base + q * BLOCK;
+        const int ic = i < a.n ? i : a.n - 1;
+        xi[q] = a.s.x[ic]; yi[q] = a.s.y[ic]; zi[q] = a.s.z[ic];
+        vxi[q] = (float)a.s.vx[ic]; vyi[q] = (float)a.s.vy[ic]; vzi[q] = (float)a.s.vz[ic];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) hi[q][c] = lo[q][c] = 0.f;
+    }
+
+    int jbeg, jend;
+    source_range<BLOCK, IPL>(a, jbeg, jend);
+    JBMX st{};
+    if (jbeg + t < jend) st = load_jbmx(a.s, jbeg + t);
+    tile[0][t] = st;
+    __syncthreads();
+    int buf = 0;
+    for (int jt = jbeg; jt < jend; jt += BLOCK) {
+        const int cnt = min(BLOCK, jend - jt);
+        const bool more = jt + BLOCK < jend;
+        if (more && jt + BLOCK + t < jend) st = load_jbmx(a.s, jt + BLOCK + t);
+        const JBMX* tb = tile[buf];
+        for (int k0 = 0; k0 < cnt; k0 += kFold) {
+            float acc[IPL][6];
+#pragma unroll
+            for (int q = 0; q < IPL; ++q)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[q][c] = 0.f;
+            if (k0 + kFold <= cnt) {
+#pragma unroll UNROLL
+                for (int k = 0; k < kFold; ++k) {
+                    const JBMX b = tb[k0 + k];
+#pragma unroll
+                    for (int q = 0; q < IPL; ++q)
+                        pair_mx<MASK>(b, jt + k0 + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q],
+                                      vzi[q], eps2, acc[q]);
+                }
+            } else {
+                for (int k = k0; k < cnt; ++k) {
+                    const JBMX b = tb[k];
+#pragma unroll
+                    for (int q = 0; q < IPL; ++q)
+                        pair_mx<MASK>(b, jt + k, base + q * BLOCK, xi[q], yi[q], zi[q], vxi[q], vyi[q], vzi[q],
+                                      eps2, acc[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < IPL; ++q)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) two_sum_fold(hi[q][c], lo[q][c], acc[q][c]);
+        }
+        if (more) tile[buf ^ 1][t] = st;
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+        const int i = base + q * BLOCK;
+        if (i < a.n) {
+            const size_t o = (size_t)blockIdx.y * a.split_stride + i;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a.out.c[c][o] = (double)hi[q][c] + (double)lo[q][c];
+        }
     }
 }
 
@@ -437,6 +587,20 @@ template <int BLOCK, int IPL>
 hipError_t launch_pair(int precision, bool mask, const PairArgs& a, int jsplit, const void* rec, hipStream_t st)
 {
     const dim3 grid((a.n + BLOCK * IPL - 1) / (BLOCK * IPL), jsplit);
+    if (!rec) {  // LDS tiles
+        if (precision == 0) {
+            if (mask)
+                k_force_f64_lds<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a);
+            else
+                k_force_f64_lds<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a);
+        } else {
+            if (mask)
+                k_force_mixed_lds<BLOCK, IPL, true><<<grid, BLOCK, 0, st>>>(a);
+            else
+                k_force_mixed_lds<BLOCK, IPL, false><<<grid, BLOCK, 0, st>>>(a);
+        }
+        return hipGetLastError();
+    }
     if (precision == 0) {
         const JB64* r = static_cast<const JB64*>(rec);
         if (mask)
@@ -484,7 +648,7 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
     a.diag = 0;
     if (shape.sym_chunk > 0) {
         if (nj > 0) return hipErrorInvalidValue;
-        return launch_force_sym(precision, s, n, eps2, shape.sym_chunk, out, rec, st, ev0, ev1);
+        return launch_force_sym(precision, s, n, eps2, shape.sym_chunk, out, st, ev0, ev1);
     }
     if (shape.jsplit > 1) {
         // partial plane for (component c, split k) starts at (c*S + k)*n
@@ -495,10 +659,10 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
         a.split_stride = 0;
     }
     const bool mask = (eps2 == 0.0);
-    const void* r = rec;
+    const void* r = shape.scalar ? rec : nullptr;
     hipError_t e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
-    if ((e = pack_source_records(precision, s, a.nj, rec, st)) != hipSuccess) return e;
+    if (shape.scalar && (e = pack_source_records(precision, s, a.nj, rec, st)) != hipSuccess) return e;
     const int key = shape.block * 10 + shape.ipl;
     switch (key) {
     case 641: e = launch_pair<64, 1>(precision, mask, a, shape.jsplit, r, st); break;
@@ -519,12 +683,10 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
 }
 
 hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
-                                  const Forces& out, void* rec, hipStream_t st)
+                                  const Forces& out, hipStream_t st)
 {
     constexpr int BLOCK = 256;
     if (B <= 0 || B % BLOCK != 0) return hipErrorInvalidValue;
-    hipError_t e = pack_source_records(precision, s, n, rec, st);
-    if (e != hipSuccess) return e;
     PairArgs a;
     a.s = s;
     a.n = n;
@@ -534,7 +696,7 @@ hipError_t launch_force_blockdiag(int precision, const State& s, int n, double e
     a.out = out;
     a.split_stride = 0;
     a.diag = B;
-    return launch_pair<BLOCK, 1>(precision, eps2 == 0.0, a, 1, rec, st);
+    return launch_pair<BLOCK, 1>(precision, eps2 == 0.0, a, 1, nullptr, st);  // LDS tiles: 512 sources each
 }
 
 hipError_t launch_predict(int n, const double* dt, double* x, double* y, double* z,

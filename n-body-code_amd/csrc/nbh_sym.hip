@@ -491,13 +491,13 @@ hipError_t launch_ens_sym(int precision, const State& s, const Forces& out, cons
 }
 
 hipError_t launch_force_sym(int precision, const State& s, int n, double eps2, int chunk, const Forces& out,
-                            void* rec, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+                            hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (!sym_supported(precision, n) || chunk < 1) return hipErrorInvalidValue;
     const int B = sym_block(precision);
     hipError_t e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
-    if ((e = launch_force_blockdiag(precision, s, n, eps2, B, out, rec, st)) != hipSuccess) return e;
+    if ((e = launch_force_blockdiag(precision, s, n, eps2, B, out, st)) != hipSuccess) return e;
     SymArgs a;
     a.s = s;
     a.n = n;

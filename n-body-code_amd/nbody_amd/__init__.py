@@ -39,10 +39,12 @@ class NBodyError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("eps2", C.c_double),
-                ("block", C.c_int), ("jsplit", C.c_int), ("ipl", C.c_int), ("pairing", C.c_int)]
+                ("block", C.c_int), ("jsplit", C.c_int), ("ipl", C.c_int), ("pairing", C.c_int),
+                ("sources", C.c_int)]
 
 
 PAIRINGS = {"directed": 0, "symmetric": 1}
+SOURCES = {"auto": 0, "lds": 1, "scalar": 2}  # enum nbh_sources
 
 
 _lib = None
@@ -78,6 +80,7 @@ def lib():
     L.nbh_launch_shape.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.nbh_launch_ipl.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.nbh_launch_pairing.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.nbh_launch_sources.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.nbh_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                C.POINTER(C.c_double)]
     L.nbh_device_info.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
@@ -173,11 +176,13 @@ def device_info(device=0):
 class Backend:
     """One nbh_ctx: a device, a stream, buffers, a resident state."""
 
-    def __init__(self, precision="fp64", eps2=1e-7, device=0, block=0, jsplit=0, ipl=0, pairing="directed"):
+    def __init__(self, precision="fp64", eps2=1e-7, device=0, block=0, jsplit=0, ipl=0, pairing="directed",
+                 sources="auto"):
         L = lib()
         self._cfg = Config(device, PRECISIONS[precision] if isinstance(precision, str) else int(precision),
                            float(eps2), int(block), int(jsplit), int(ipl),
-                           PAIRINGS[pairing] if isinstance(pairing, str) else int(pairing))
+                           PAIRINGS[pairing] if isinstance(pairing, str) else int(pairing),
+                           SOURCES[sources] if isinstance(sources, str) else int(sources))
         h = C.c_void_p()
         _check(L.nbh_create(C.byref(self._cfg), C.byref(h)))
         self._h = h
@@ -200,10 +205,12 @@ class Backend:
     def __exit__(self, *exc):
         self.close()
 
-    def configure(self, precision=None, eps2=None, block=None, jsplit=None, ipl=None, pairing=None):
+    def configure(self, precision=None, eps2=None, block=None, jsplit=None, ipl=None, pairing=None, sources=None):
         c = self._cfg
         if pairing is not None:
             c.pairing = PAIRINGS[pairing] if isinstance(pairing, str) else int(pairing)
+        if sources is not None:
+            c.sources = SOURCES[sources] if isinstance(sources, str) else int(sources)
         if ipl is not None:
             c.ipl = int(ipl)
         if precision is not None:
@@ -231,8 +238,12 @@ class Backend:
         _check(lib().nbh_launch_shape(self._h, int(n), C.byref(b), C.byref(s), C.byref(w)))
         _check(lib().nbh_launch_ipl(self._h, int(n), C.byref(q)))
         _check(lib().nbh_launch_pairing(self._h, int(n), C.byref(pr)))
+        src = C.c_int()
+        _check(lib().nbh_launch_sources(self._h, int(n), C.byref(src)))
+        names = {SOURCES["lds"]: "lds", SOURCES["scalar"]: "scalar"}  # 0: the symmetric kernel
         return {"block": b.value, "ipl": q.value, "jsplit": s.value, "workgroups": w.value,
-                "pairing": "symmetric" if pr.value == 1 else "directed"}
+                "pairing": "symmetric" if pr.value == 1 else "directed",
+                "sources": names.get(src.value, "none")}
 
     def autotune(self, n):
         """Times launch shapes for n bodies and keeps the fastest for n's size bucket."""

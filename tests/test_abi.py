@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(nb):
 
 
 def test_version(nb):
-    assert nb.lib().nbh_version() == 101
+    assert nb.lib().nbh_version() == 102
 
 
 def test_reference_ic_matches_reference_fixture(nb):
@@ -93,9 +93,10 @@ def test_bad_config_rejected_before_device_use(nb):
     h = C.c_void_p()
     for cfg in (nb.Config(0, 7, 1e-7, 0, 0), nb.Config(0, 0, -1.0, 0, 0),
                 nb.Config(0, 0, 1e-7, 96, 0), nb.Config(0, 0, 1e-7, 0, -2),
-                nb.Config(0, 1, 1e-7, 0, 0, 0, 2)):    # unknown pairing
+                nb.Config(0, 1, 1e-7, 0, 0, 0, 2),     # unknown pairing
+                nb.Config(0, 1, 1e-7, 0, 0, 0, 0, 3)):  # unknown source path
         assert L.nbh_create(C.byref(cfg), C.byref(h)) == nb.ERR_ARG
-    assert b"pairing must be" in L.nbh_last_error()
+    assert b"sources must be" in L.nbh_last_error()
     assert L.nbh_create(None, C.byref(h)) == nb.ERR_ARG
 
 

@@ -70,17 +70,49 @@ def test_fp64_ragged_sizes(be64, oracle, nb, n):
         check64(f, ref)
 
 
+@pytest.mark.parametrize("sources", ["lds", "scalar"])
 @pytest.mark.parametrize("block", [64, 128, 256])
 @pytest.mark.parametrize("jsplit", [1, 2, 7, 0])
 @pytest.mark.parametrize("ipl", [1, 2])
-def test_fp64_launch_shapes_agree(be64, oracle, nb, block, jsplit, ipl):
-    _banner(f"shape block={block} jsplit={jsplit} ipl={ipl}")
+def test_fp64_launch_shapes_agree(be64, oracle, nb, block, jsplit, ipl, sources):
+    _banner(f"shape block={block} jsplit={jsplit} ipl={ipl} sources={sources}")
     s = nb.plummer(3000, 77)
-    be64.configure(eps2=1e-3, block=block, jsplit=jsplit, ipl=ipl)
+    be64.configure(eps2=1e-3, block=block, jsplit=jsplit, ipl=ipl, sources=sources)
     try:
+        assert be64.launch_shape(3000)["sources"] == sources
         check64(be64.force(s), oracle.force(s, 1e-3))
     finally:
-        be64.configure(block=0, jsplit=0, ipl=0)
+        be64.configure(block=0, jsplit=0, ipl=0, sources="auto")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n,eps2", [(1, 1e-4), (65, 1e-4), (1000, 0.0), (3001, 1e-3), (20000, 1e-5)])
+def test_source_paths_bitwise_identical(nb, precision, n, eps2):
+    """LDS tiles and scalar loads of packed records feed each target the same
+    sources in the same order: the forces agree bit for bit, for every shape."""
+    s = nb.plummer(n, 90 + n)
+    with nb.Backend(precision, eps2) as b:
+        for block, ipl, jsplit in ((64, 1, 1), (128, 2, 3), (256, 1, 0)):
+            b.configure(block=block, ipl=ipl, jsplit=jsplit, sources="lds")
+            f_lds = b.force(s)
+            b.configure(sources="scalar")
+            f_sc = b.force(s)
+            assert np.array_equal(f_lds, f_sc), (block, ipl, jsplit)
+
+
+def test_default_source_path_by_size(nb):
+    """Without tuning: LDS tiles for small n, scalar loads from 65536 (mixed)
+    and 262144 (FP64) bodies; the symmetric kernel reports none."""
+    with nb.Backend("mixed", 1e-4) as b:
+        assert b.launch_shape(4096)["sources"] == "lds"
+        assert b.launch_shape(65536)["sources"] == "scalar"
+    with nb.Backend("fp64", 1e-4) as b:
+        assert b.launch_shape(131072)["sources"] == "lds"
+        assert b.launch_shape(262144)["sources"] == "scalar"
+        b.configure(sources="lds")
+        assert b.launch_shape(262144)["sources"] == "lds"
+    with nb.Backend("mixed", 1e-4, pairing="symmetric") as b:
+        assert b.launch_shape(4096)["sources"] == "none"
 
 
 @pytest.mark.parametrize("ipl", [1, 2])
@@ -157,36 +189,41 @@ def check_mixed(f, ref):
 
 # the mixed directed kernel reads its sources from packed records at
 # wave-uniform indices: ragged ends of fold runs, splits and targets
+@pytest.mark.parametrize("sources", ["lds", "scalar"])
 @pytest.mark.parametrize("n", [1, 2, 3, 63, 65, 257, 1000, 1023, 1025, 4097])
-def test_mixed_ragged_sizes(bemx, oracle, nb, n):
-    _banner(f"mixed ragged n={n}")
+def test_mixed_ragged_sizes(bemx, oracle, nb, n, sources):
+    _banner(f"mixed ragged n={n} sources={sources}")
     s = nb.plummer(n, 2000 + n)
-    bemx.configure(eps2=1e-4, block=0, jsplit=0, ipl=0)
+    bemx.configure(eps2=1e-4, block=0, jsplit=0, ipl=0, sources=sources)
     f = bemx.force(s)
+    bemx.configure(sources="auto")
     if n == 1:
         assert np.all(f == 0.0)
     else:
         check_mixed(f, oracle.force(s, 1e-4))
 
 
+@pytest.mark.parametrize("sources", ["lds", "scalar"])
 @pytest.mark.parametrize("block", [64, 128, 256])
 @pytest.mark.parametrize("jsplit", [1, 2, 7, 0])
 @pytest.mark.parametrize("ipl", [1, 2])
-def test_mixed_launch_shapes_agree(bemx, oracle, nb, block, jsplit, ipl):
-    _banner(f"mixed shape block={block} jsplit={jsplit} ipl={ipl}")
+def test_mixed_launch_shapes_agree(bemx, oracle, nb, block, jsplit, ipl, sources):
+    _banner(f"mixed shape block={block} jsplit={jsplit} ipl={ipl} sources={sources}")
     s = nb.plummer(3000, 77)
-    bemx.configure(eps2=1e-3, block=block, jsplit=jsplit, ipl=ipl)
+    bemx.configure(eps2=1e-3, block=block, jsplit=jsplit, ipl=ipl, sources=sources)
     try:
         check_mixed(bemx.force(s), oracle.force(s, 1e-3))
     finally:
-        bemx.configure(block=0, jsplit=0, ipl=0)
+        bemx.configure(block=0, jsplit=0, ipl=0, sources="auto")
 
 
-def test_mixed_zero_softening(bemx, oracle, nb):
+@pytest.mark.parametrize("sources", ["lds", "scalar"])
+def test_mixed_zero_softening(bemx, oracle, nb, sources):
     """eps2 = 0: the self pair is dropped in the mixed kernel too."""
     s = nb.plummer(777, 8)
-    bemx.configure(eps2=0.0)
+    bemx.configure(eps2=0.0, sources=sources)
     f = bemx.force(s)
+    bemx.configure(sources="auto")
     assert np.isfinite(f).all()
     check_mixed(f, oracle.force(s, 0.0))
 
@@ -277,8 +314,9 @@ def test_force_device_pointers(be64, oracle, nb):
 
 
 def test_force_device_pointers_mixed(bemx, oracle, nb):
-    """nbh_force_device in mixed precision: the context packs the caller's
-    source arrays into its own record buffer, which grows with n."""
+    """nbh_force_device in mixed precision with scalar-load sources: the
+    context packs the caller's source arrays into its own record buffer,
+    which grows with n."""
     import ctypes as C
     hip = C.CDLL("libamdhip64.so.7")
     for n in (1500, 4100, 700):
@@ -289,7 +327,7 @@ def test_force_device_pointers_mixed(bemx, oracle, nb):
         assert hip.hipMalloc(C.byref(d_out), C.c_size_t(out.nbytes)) == 0
         try:
             assert hip.hipMemcpy(d_in, s.ctypes.data_as(C.c_void_p), C.c_size_t(s.nbytes), 1) == 0
-            bemx.configure(eps2=1e-4, block=0, jsplit=0, ipl=0)
+            bemx.configure(eps2=1e-4, block=0, jsplit=0, ipl=0, sources="scalar")
             col = lambda base, k: C.c_void_p(base.value + 8 * n * k)  # noqa: E731
             rc = nb.lib().nbh_force_device(bemx._h, n, *[col(d_in, k) for k in range(7)],
                                            *[col(d_out, k) for k in range(6)], None)
@@ -301,6 +339,7 @@ def test_force_device_pointers_mixed(bemx, oracle, nb):
             hip.hipFree(d_out)
         check_mixed(out, oracle.force(s, 1e-4))
         assert np.array_equal(out, bemx.force(s))  # same kernel through the host-array call
+    bemx.configure(sources="auto")
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
