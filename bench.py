@@ -94,12 +94,14 @@ def measured_traffic():
 
 
 def kname(prec, res):
-    """Pair-kernel instance of a run: k_force_{f64,mixed}<block,targets per lane>
-    or the symmetric kernel k_force_sym<SymF64|SymMixed,targets per lane,waves>."""
+    """Pair-kernel instance of a run: k_force_{f64,mixed}[_lds]<block,targets per
+    lane> (scalar-load or LDS-tile sources) or the symmetric kernel
+    k_force_sym<SymF64|SymMixed,targets per lane,waves>."""
     sh = res["shape"]
     if sh.get("pairing") == "symmetric":
         return f"k_force_sym<{'SymF64' if prec == 'fp64' else 'SymMixed'},{sh['ipl']},{sh['block'] // 64}>"
-    return f"k_force_{'f64' if prec == 'fp64' else 'mixed'}<{sh['block']},{sh['ipl']}>"
+    lds = "_lds" if sh.get("sources") == "lds" else ""
+    return f"k_force_{'f64' if prec == 'fp64' else 'mixed'}{lds}<{sh['block']},{sh['ipl']}>"
 
 
 def kernel_flop(shape):
@@ -167,10 +169,26 @@ def run_precision(nb, prec, state, steps, warmup, log, pairing="directed"):
     return res, f0
 
 
+def traffic_of(traffic, kern):
+    """Measured HBM bytes per launch of `kern`, or of the same kernel with
+    another block size (the autotuner may pick either from run to run);
+    returns (bytes, instance measured) or (None, None)."""
+    if kern in traffic:
+        return traffic[kern], kern
+    head, tail = kern.split("<", 1)
+    ipl = tail.rstrip(">").split(",")[-1]
+    for k, v in sorted(traffic.items()):
+        kh, kt = k.split("<", 1)
+        if kh == head and kt.rstrip(">").split(",")[-1] == ipl:
+            return v, k
+    return None, None
+
+
 def roofline(prec, res, traffic):
+    tb, tk = traffic_of(traffic, kname(prec, res))
     out = {"bound": "valu", "achieved": res["kernel_tflops"], "peak": PEAK[prec],
            "unit": "TFLOP/s", "frac": res["kernel_tflops"] / PEAK[prec],
-           "traffic": traffic.get(kname(prec, res)),
+           "traffic": tb, "traffic_kernel": tk,
            "kernel": kname(prec, res), "kernel_ms": res["kernel_ms"],
            "flop_per_launch": res["kernel_flop"],
            "flop_convention": "60 flop per directed pair, N^2 directed pairs per evaluation"}
