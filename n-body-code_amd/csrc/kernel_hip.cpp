@@ -25,7 +25,7 @@ void force_backend_configure(const ForceBackend& fb)
 {
     g_fb = fb;
     if (g_ctx) {
-        nbh_config cfg{fb.device, fb.precision, fb.eps2, 0, 0, 0, fb.pairing, NBH_SRC_AUTO};
+        nbh_config cfg{fb.device, fb.precision, fb.eps2, 0, 0, 0, fb.pairing, fb.sources};
         if (nbh_configure(g_ctx, &cfg) != NBH_OK) die("nbh_configure");
     }
 }
@@ -33,7 +33,7 @@ void force_backend_configure(const ForceBackend& fb)
 nbh_ctx* force_backend_context()
 {
     if (!g_ctx) {
-        nbh_config cfg{g_fb.device, g_fb.precision, g_fb.eps2, 0, 0, 0, g_fb.pairing, NBH_SRC_AUTO};
+        nbh_config cfg{g_fb.device, g_fb.precision, g_fb.eps2, 0, 0, 0, g_fb.pairing, g_fb.sources};
         if (nbh_create(&cfg, &g_ctx) != NBH_OK) die("nbh_create");
         if (nbh_device_info(g_fb.device, &g_cus, nullptr, nullptr, 0) != NBH_OK) die("nbh_device_info");
     }

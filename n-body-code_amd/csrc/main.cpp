@@ -18,7 +18,8 @@ constexpr int kCol = 15;
 const char* kUsage =
     "Usage: ./nbody_app [--N <value>] [--Ncycle <value>]\n"
     "  extensions: [--eps <softening>] [--dt <fixed step>] [--ic ref|plummer] [--seed <int>]\n"
-    "              [--precision fp64|mixed] [--pairing directed|symmetric] [--resident] [--graph]\n"
+    "              [--precision fp64|mixed] [--pairing directed|symmetric] [--sources auto|lds|scalar]\n"
+    "              [--resident] [--graph]\n"
     "              [--energy compat|physical]\n"
     "              [--device <ordinal>] [--load <state file>] [--save <state file>]";
 
@@ -76,6 +77,12 @@ int main(int argc, char* argv[])
                 if (p == "directed") fb.pairing = NBH_PAIR_DIRECTED;
                 else if (p == "symmetric") fb.pairing = NBH_PAIR_SYMMETRIC;
                 else throw std::invalid_argument("pairing must be directed or symmetric.");
+            } else if (arg == "--sources" && has) {
+                const std::string p = argv[++i];
+                if (p == "auto") fb.sources = NBH_SRC_AUTO;
+                else if (p == "lds") fb.sources = NBH_SRC_LDS;
+                else if (p == "scalar") fb.sources = NBH_SRC_SCALAR;
+                else throw std::invalid_argument("sources must be auto, lds or scalar.");
             } else if (arg == "--energy" && has) {
                 energy = argv[++i];
                 if (energy != "compat" && energy != "physical")

@@ -46,6 +46,7 @@ struct ForceBackend {
     int precision = NBH_FP64;
     double eps2 = 1.0e-7;  // the reference's EPS2 (include/kernel.h:13)
     int pairing = NBH_PAIR_DIRECTED;
+    int sources = NBH_SRC_AUTO;  // how the directed kernels read their sources
 };
 
 // Sets (or changes) the backend; the context is opened on first use.

@@ -26,6 +26,7 @@ def run(*args, timeout=120):
     (["--precision", "fp16"], "precision must be"),
     (["--dt", "0"], "dt must be positive"),
     (["--pairing", "both"], "pairing must be directed or symmetric"),
+    (["--sources", "smem"], "sources must be auto, lds or scalar"),
 ])
 def test_bad_arguments_exit_nonzero(args, needle):
     r = run(*args)
@@ -66,6 +67,19 @@ def test_symmetric_pairing_run(precision, tol):
     ea, eb = energies(a.stdout), energies(b.stdout)
     assert abs(ea[0] - eb[0]) <= 1e-12 * abs(ea[0])
     assert abs(ea[1] - eb[1]) <= tol * abs(ea[1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_source_paths_same_run(precision):
+    """--sources lds and --sources scalar: the same forces bit for bit, so the
+    same printed energies."""
+    common = ["--N", "3000", "--Ncycle", "3", "--ic", "plummer", "--seed", "8", "--eps", "0.01",
+              "--dt", "0.001", "--energy", "physical", "--precision", precision]
+    a = run(*common, "--sources", "lds")
+    b = run(*common, "--sources", "scalar")
+    assert a.returncode == 0 and b.returncode == 0, a.stderr + b.stderr
+    assert energies(a.stdout) == energies(b.stdout)
 
 
 @pytest.mark.gpu
