@@ -183,17 +183,23 @@ void drop_graph(nbh_ctx* c)
     c->g_n = -1;
 }
 
+// The split partials and packed source records that a force evaluation of
+// shape sh needs; done before a graph capture too, so that no allocation
+// happens while the stream is captured.
+int reserve_force_buffers(nbh_ctx* c, const nbh::Shape& sh, int n)
+{
+    if (sh.jsplit > 1) NBH_HIP(c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n));
+    if (sh.scalar) NBH_HIP(c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)));
+    return NBH_OK;
+}
+
 // One force evaluation on device arrays, enqueued on st.
 int enqueue_force(nbh_ctx* c, const nbh::State& s, int n, const nbh::Forces& out,
                   hipStream_t st, bool allow_events)
 {
     const nbh::Shape sh = pick_shape(c, n);
-    double* part = nullptr;
-    if (sh.jsplit > 1) {
-        NBH_HIP(c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n));
-        part = c->partial.d();
-    }
-    if (sh.scalar) NBH_HIP(c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)));
+    if (int rc = reserve_force_buffers(c, sh, n)) return rc;
+    double* part = sh.jsplit > 1 ? c->partial.d() : nullptr;
     EventPair ep;
     const bool timed = c->profiling && allow_events;
     if (timed) {
@@ -640,6 +646,7 @@ int enqueue_step(nbh_ctx* c, bool allow_events)
 int build_graph(nbh_ctx* c, int steps)
 {
     drop_graph(c);
+    if (int rc = reserve_force_buffers(c, pick_shape(c, c->n), c->n)) return rc;
     NBH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = NBH_OK;
     for (int k = 0; k < steps && rc == NBH_OK; ++k) rc = enqueue_step(c, false);

@@ -122,6 +122,26 @@ def test_graph_replay_matches_eager(nb):
         assert np.array_equal(g.download()[0], e.download()[0])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_after_shape_change(nb, precision):
+    """A graph captured after the launch shape changed (more j-splits, scalar
+    sources) needs larger partial and record buffers than the eager steps
+    before it: they are reserved before the capture, and every replay matches
+    eager steps of the same shape bit for bit."""
+    s = nb.plummer(3000, 12)
+    with nb.Backend(precision, 1e-4) as g, nb.Backend(precision, 1e-4) as e:
+        for b in (g, e):
+            b.configure(block=64, ipl=1, jsplit=1, sources="lds")
+            b.upload(s)
+            b.set_dt(1 / 1024)
+            b.step(4)
+            b.configure(block=128, ipl=2, jsplit=24, sources="scalar")
+        for r in range(3):
+            g.step(8, graph=True)
+            e.step(8, graph=False)
+            assert np.array_equal(g.download()[0], e.download()[0]), f"replay {r}"
+
+
 def test_adaptive_graph_matches_eager(nb):
     s = nb.plummer(1500, 6)
     res = []
