@@ -188,6 +188,8 @@ def roofline(prec, res, traffic):
     tb, tk = traffic_of(traffic, kname(prec, res))
     out = {"bound": "valu", "achieved": res["kernel_tflops"], "peak": PEAK[prec],
            "unit": "TFLOP/s", "frac": res["kernel_tflops"] / PEAK[prec],
+           "bound_note": "vector-ALU FLOP roofline (MI355X vector peak: FP64 78.6, FP32 157.3 TFLOP/s): "
+                         "the softened r^-3 pair sum is VALU work, HBM-bound by no measure and no MFMA contraction",
            "traffic": tb, "traffic_kernel": tk,
            "kernel": kname(prec, res), "kernel_ms": res["kernel_ms"],
            "flop_per_launch": res["kernel_flop"],
