@@ -64,6 +64,9 @@ __device__ __forceinline__ void source_range(const PairArgs& a, int& jbeg, int& 
     }
 }
 
+// LDS-tile kernels.  BLOCK lanes per workgroup, IPL targets per lane (lane t
+// holds targets base + t + q*BLOCK, q < IPL), so each broadcast source
+// record read from LDS feeds IPL pair evaluations.
 template <int BLOCK, int IPL, bool MASK>
 __global__ __launch_bounds__(BLOCK) void k_force_f64_lds(PairArgs a)
 {
@@ -129,6 +132,8 @@ __global__ __launch_bounds__(BLOCK) void k_force_f64_lds(PairArgs a)
     }
 }
 
+// Terms summed over runs of kFold sources from the split's first source (a
+// tile holds whole runs), each run folded into the (hi, lo) totals.
 template <int BLOCK, int IPL, bool MASK>
 __global__ __launch_bounds__(BLOCK) void k_force_mixed_lds(PairArgs a)
 {
