@@ -226,9 +226,10 @@ __global__ __launch_bounds__(kOBlock) void k_pack_mixed(State s, int n, JBMX* __
     if (j < n) rec[j] = load_jbmx(s, j);
 }
 
-// BLOCK lanes per workgroup, IPL targets per lane (lane t holds targets
-// base + t + q*BLOCK, q < IPL), so each source record (from `rec`, at a
-// wave-uniform index: scalar loads) feeds IPL pair evaluations.  Each target
+// Scalar-load kernels.  BLOCK lanes per workgroup, IPL targets per lane
+// (lane t holds targets base + t + q*BLOCK, q < IPL), so each source record
+// (from `rec`, at a wave-uniform index: scalar loads) feeds IPL pair
+// evaluations.  Each target
 // sums its sources in order, whatever BLOCK and IPL.
 template <int BLOCK, int IPL, bool MASK>
 __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a, const JB64* __restrict__ rec)
