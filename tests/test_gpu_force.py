@@ -251,19 +251,25 @@ def test_mixed_vs_reference_f32_fixture(bemx):
     assert np.median(e) <= 1e-6
 
 
-def test_large_sample_vs_reference_fixture(be64, bemx, nb):
+@pytest.mark.parametrize("sources", ["lds", "scalar"])
+def test_large_sample_vs_reference_fixture(be64, bemx, nb, sources):
     """N=102400 Plummer seed 2 (the 102,400-body configuration's sphere) at the
-    reference's softening, every 100th body against the reference."""
+    reference's softening, every 100th body against the reference, with either
+    source path."""
     g = golden("plummer_n102400_sample.npz")
     s = nb.plummer(int(g["n"]), int(g["seed"]))
     idx = g["idx"]
-    be64.configure(eps2=EPS2_REF)
-    f = be64.force(s)
-    check64(f[:, idx], g["force64"])
-    bemx.configure(eps2=EPS2_REF)
-    fm = bemx.force(s)
+    try:
+        be64.configure(eps2=EPS2_REF, sources=sources)
+        f = be64.force(s)
+        check64(f[:, idx], g["force64"])
+        bemx.configure(eps2=EPS2_REF, sources=sources)
+        fm = bemx.force(s)
+    finally:
+        be64.configure(sources="auto")
+        bemx.configure(sources="auto")
     e = body_relerr(fm[:3, idx], g["force64"][:3])
-    _banner(f"N=102400 mixed median {np.median(e):.2e}")
+    _banner(f"N=102400 mixed ({sources}) median {np.median(e):.2e}")
     assert np.median(e) <= 1e-6
 
 
