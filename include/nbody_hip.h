@@ -119,7 +119,10 @@ int nbh_force(nbh_ctx *ctx, int n,
  * No counterpart in the reference (its arrays are host memory); it lets a
  * caller that already holds device buffers skip the copies.  With
  * NBH_PAIR_SYMMETRIC the outputs receive FP64 atomic adds, so they must be
- * ordinary device allocations (hipMalloc), not host or managed memory. */
+ * ordinary device allocations (hipMalloc), not host or managed memory.
+ * The context's j-split partials and packed source records are shared by
+ * all of its calls: two evaluations that may run at the same time (on
+ * different streams) need two contexts. */
 int nbh_force_device(nbh_ctx *ctx, int n,
                      const double *x, const double *y, const double *z,
                      const double *m, const double *vx, const double *vy,

@@ -64,7 +64,11 @@ int get_events(nbh_ctx* c, EventPair* ep)
         return NBH_OK;
     }
     NBH_HIP(hipEventCreate(&ep->a));
-    NBH_HIP(hipEventCreate(&ep->b));
+    const hipError_t e = hipEventCreate(&ep->b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(ep->a);
+        return fail(NBH_ERR_HIP, "hipEventCreate failed: %s", hipGetErrorString(e));
+    }
     return NBH_OK;
 }
 
@@ -208,8 +212,8 @@ int enqueue_force(nbh_ctx* c, const nbh::State& s, int n, const nbh::Forces& out
     }
     hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, out, part, c->rec.p, st,
                                      timed ? ep.a : nullptr, timed ? ep.b : nullptr);
+    if (timed) (e == hipSuccess ? c->ev_pending : c->ev_free).push_back(ep);
     if (e != hipSuccess) return fail(NBH_ERR_HIP, "force launch failed: %s", hipGetErrorString(e));
-    if (timed) c->ev_pending.push_back(ep);
     return NBH_OK;
 }
 
@@ -487,36 +491,42 @@ int nbh_force(nbh_ctx* c, int n, const double* x, const double* y, const double*
         NBH_HIP(hipMemcpyAsync(c->in.d() + (size_t)k * n, ins[k], nb, hipMemcpyHostToDevice, c->stream));
     const nbh::State s = state_of(c->in.d(), n);
     const nbh::Forces f = forces_of(c->out.d(), n);
-    // the host call times its own kernels regardless of the profiling switch
+    const nbh::Shape sh = pick_shape(c, n);
+    if (sh.jsplit > 1 && c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n) != hipSuccess)
+        return fail(NBH_ERR_ALLOC, "device allocation of split partials failed");
+    if (sh.scalar && c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)) != hipSuccess)
+        return fail(NBH_ERR_ALLOC, "device allocation of source records failed");
+    double* part = sh.jsplit > 1 ? c->partial.d() : nullptr;
+    NBH_HIP(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
+    // the host call times its own kernels regardless of the profiling switch;
+    // the event pair goes back to the free list on every path below
     EventPair ep;
     int rc = get_events(c, &ep);
     if (rc) return rc;
-    const nbh::Shape sh = pick_shape(c, n);
-    double* part = nullptr;
-    if (sh.jsplit > 1) {
-        if (c->partial.reserve(sizeof(double) * 6 * (size_t)sh.jsplit * n) != hipSuccess)
-            return fail(NBH_ERR_ALLOC, "device allocation of split partials failed");
-        part = c->partial.d();
-    }
-    if (sh.scalar && c->rec.reserve(nbh::source_record_bytes(c->cfg.precision, n)) != hipSuccess)
-        return fail(NBH_ERR_ALLOC, "device allocation of source records failed");
-    NBH_HIP(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
+    int flag = 0;
+    float ms = 0.f;
     hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, f, part, c->rec.p, c->stream,
                                      ep.a, ep.b);
-    if (e != hipSuccess) {
-        c->ev_free.push_back(ep);
-        return fail(NBH_ERR_HIP, "force launch failed: %s", hipGetErrorString(e));
+    const char* what = "force launch";
+    if (e == hipSuccess) {
+        what = "finite check";
+        e = nbh::launch_check_finite(n, f, static_cast<int*>(c->flag.p), c->stream);
     }
-    e = nbh::launch_check_finite(n, f, static_cast<int*>(c->flag.p), c->stream);
-    if (e != hipSuccess) return fail(NBH_ERR_HIP, "finite check failed: %s", hipGetErrorString(e));
-    for (int k = 0; k < 6; ++k)
-        NBH_HIP(hipMemcpyAsync(outs[k], c->out.d() + (size_t)k * n, nb, hipMemcpyDeviceToHost, c->stream));
-    int flag = 0;
-    NBH_HIP(hipMemcpyAsync(&flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    NBH_HIP(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
-    NBH_HIP(hipEventElapsedTime(&ms, ep.a, ep.b));
+    for (int k = 0; k < 6 && e == hipSuccess; ++k) {
+        what = "result copy";
+        e = hipMemcpyAsync(outs[k], c->out.d() + (size_t)k * n, nb, hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, c->flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) {
+        what = "stream synchronize";
+        e = hipStreamSynchronize(c->stream);
+    }
+    if (e == hipSuccess) {
+        what = "kernel timing";
+        e = hipEventElapsedTime(&ms, ep.a, ep.b);
+    }
     c->ev_free.push_back(ep);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
     if (c->profiling) {
         c->force_ms += ms;
         c->force_launches += 1;

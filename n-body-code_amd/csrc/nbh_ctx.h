@@ -98,4 +98,27 @@ struct nbh_ctx {
 namespace nbh {
 int get_events(nbh_ctx* c, EventPair* ep);
 int settle_events(nbh_ctx* c);
+
+// An event pair taken from the context: handed to the pending list once
+// both events are recorded (pending()), else returned to the free list when
+// the guard goes out of scope, so that error returns do not lose it.
+struct EventGuard {
+    nbh_ctx* c = nullptr;
+    EventPair ep;
+    int take(nbh_ctx* ctx)
+    {
+        int rc = get_events(ctx, &ep);
+        if (rc == 0) c = ctx;
+        return rc;
+    }
+    void pending()
+    {
+        c->ev_pending.push_back(ep);
+        c = nullptr;
+    }
+    ~EventGuard()
+    {
+        if (c) c->ev_free.push_back(ep);
+    }
+};
 }  // namespace nbh

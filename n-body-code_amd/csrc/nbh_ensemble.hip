@@ -475,20 +475,20 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
     a.partial = e->partial.d();
     nbh::k_zero_u32<<<1, 64, 0, c->stream>>>(a.counter);
     NBH_HIP(hipGetLastError());
-    nbh::EventPair ep;
+    nbh::EventGuard ev;
     const bool timed = c->profiling && allow_events;
     if (timed) {
-        int rc = nbh::get_events(c, &ep);
+        int rc = ev.take(c);
         if (rc) return rc;
-        NBH_HIP(hipEventRecord(ep.a, c->stream));
+        NBH_HIP(hipEventRecord(ev.ep.a, c->stream));
     }
     const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, e->maxtile, e->any_zero_eps, e->list_ipl);
     if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d", e->maxtile);
     void* args[] = {&a};
     NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, c->stream));
     if (timed && e->nsym == 0) {
-        NBH_HIP(hipEventRecord(ep.b, c->stream));
-        c->ev_pending.push_back(ep);
+        NBH_HIP(hipEventRecord(ev.ep.b, c->stream));
+        ev.pending();
     }
     if (e->nreduce > 0) {
         nbh::k_ens_reduce<<<e->nreduce, nbh::kLanes, 0, c->stream>>>(static_cast<const EnsReduce*>(e->reduces.p),
@@ -499,8 +499,8 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
         NBH_HIP(nbh::launch_ens_sym(c->cfg.precision, a.s, out, static_cast<const nbh::SymItem*>(e->symitems.p),
                                     e->nsym, c->stream));
         if (timed) {
-            NBH_HIP(hipEventRecord(ep.b, c->stream));
-            c->ev_pending.push_back(ep);
+            NBH_HIP(hipEventRecord(ev.ep.b, c->stream));
+            ev.pending();
         }
     }
     return NBH_OK;
