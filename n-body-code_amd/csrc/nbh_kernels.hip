@@ -488,12 +488,19 @@ __global__ __launch_bounds__(kOBlock) void k_initial_dt(int n, Forces f, double 
 }
 
 // ----------------------------------------------------------- energy -----
-// Kinetic and potential energy in FP64 (src/utils.cpp:125-153).  Per target
-// body i the potential sum phi_i runs over source tiles like the pair
-// kernels.  compat = 0: U = -1/2 sum_i m_i sum_{j != i} m_j / sqrt(r^2+eps2).
-// compat = 1: the reference's own definition, U = -sum_i sum_{j>i} m_j^2 /
-// sqrt(r^2 + eps2) (src/utils.cpp:148).  Block sums are written per block
-// and added in block order by k_energy_final: deterministic.
+// Kinetic and potential energy in FP64 (src/utils.cpp:125-153), each
+// unordered pair evaluated once, as the reference's j > i loop does.
+// compat = 0: U = -sum_{i<j} m_i m_j / sqrt(r^2 + eps2).  compat = 1: the
+// reference's own definition, U = -sum_i sum_{j>i} m_j^2 / sqrt(r^2 + eps2)
+// (src/utils.cpp:148), i.e. each pair weighted by the squared mass of its
+// higher-indexed body.
+//
+// Workgroup b holds targets b*kEBlock .. and visits the source tiles
+// b + k (mod nb), k = 0 .. (nb-1)/2, plus k = nb/2 when nb is even and
+// b < nb/2: tile pairs are taken once each and every workgroup does about
+// nb/2 tiles.  In its own tile (k = 0) a target takes the sources j > i.
+// Block sums are written per block and added in block order by
+// k_energy_final: deterministic.
 constexpr int kEBlock = 256;
 
 template <bool COMPAT>
@@ -502,33 +509,51 @@ __global__ __launch_bounds__(kEBlock) void k_energy(State s, int n, double eps2,
     __shared__ double4 tile[kEBlock];
     __shared__ double red[2][kEBlock / 64];
     const int t = threadIdx.x;
-    const int i = blockIdx.x * kEBlock + t;
+    const int b = blockIdx.x;
+    const int nb = gridDim.x;
+    const int i = b * kEBlock + t;
     const int ic = i < n ? i : n - 1;
     const double xi = s.x[ic], yi = s.y[ic], zi = s.z[ic];
-    double phi = 0.0;
-    for (int jt = 0; jt < n; jt += kEBlock) {
+    const int kmax = (nb - 1) / 2 + ((nb % 2 == 0 && b < nb / 2) ? 1 : 0);
+    double phi = 0.0;  // sum over sources of w_j / r: w_j = m_j, or m_j^2 (compat, j > i)
+    double rin = 0.0;  // compat: sum of 1 / r over sources j < i (weighted by m_i^2 at the end)
+    for (int k = 0; k <= kmax; ++k) {
+        const int q = b + k < nb ? b + k : b + k - nb;
+        const int jt = q * kEBlock;
         const int j = jt + t;
-        double4 b = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (j < n) b = make_double4(s.x[j], s.y[j], s.z[j], s.m[j]);
+        double4 src = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (j < n) src = make_double4(s.x[j], s.y[j], s.z[j], COMPAT ? s.m[j] * s.m[j] : s.m[j]);
         __syncthreads();
-        tile[t] = b;
+        tile[t] = src;
         __syncthreads();
         const int cnt = min(kEBlock, n - jt);
-        for (int k = 0; k < cnt; ++k) {
-            const double4 q = tile[k];
-            const int jj = jt + k;
-            const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
-            double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
-            const bool keep = COMPAT ? (jj > i) : (jj != i);
-            r2 = keep ? r2 : 1.0;
-            const double w = keep ? (COMPAT ? q.w * q.w : q.w) : 0.0;
-            phi = fma(w, rsqrt_f64(r2), phi);
+        if (k == 0) {
+            for (int kk = 0; kk < cnt; ++kk) {
+                const double4 p = tile[kk];
+                const double dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
+                double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+                const bool keep = jt + kk > i;
+                r2 = keep ? r2 : 1.0;
+                phi = fma(keep ? p.w : 0.0, rsqrt_f64(r2), phi);
+            }
+        } else if (COMPAT && q < b) {  // wrapped: every source has a lower index
+            for (int kk = 0; kk < cnt; ++kk) {
+                const double4 p = tile[kk];
+                const double dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
+                rin += rsqrt_f64(fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2))));
+            }
+        } else {
+            for (int kk = 0; kk < cnt; ++kk) {
+                const double4 p = tile[kk];
+                const double dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
+                phi = fma(p.w, rsqrt_f64(fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)))), phi);
+            }
         }
     }
     double pot = 0.0, kin = 0.0;
     if (i < n) {
         const double mi = s.m[i];
-        pot = COMPAT ? -phi : -0.5 * mi * phi;
+        pot = COMPAT ? -fma(mi * mi, rin, phi) : -mi * phi;
         const double ux = s.vx[i], uy = s.vy[i], uz = s.vz[i];
         kin = 0.5 * mi * (ux * ux + uy * uy + uz * uz);
     }
@@ -547,8 +572,8 @@ __global__ __launch_bounds__(kEBlock) void k_energy(State s, int n, double eps2,
             k += red[0][w];
             p += red[1][w];
         }
-        work[2 * blockIdx.x] = k;
-        work[2 * blockIdx.x + 1] = p;
+        work[2 * b] = k;
+        work[2 * b + 1] = p;
     }
 }
 

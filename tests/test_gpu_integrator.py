@@ -173,6 +173,25 @@ def test_energy_physical_vs_oracle(nb, oracle):
         assert pot == pytest.approx(oracle.potential(s, eps2, nthreads=16), rel=1e-12)
 
 
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 513, 769, 1000, 1281, 2560])
+def test_energy_ragged_sizes(nb, oracle, n):
+    """Odd and even tile counts (the pair schedule takes each tile pair once)
+    and partial last tiles, unequal masses, both potential definitions."""
+    _banner(f"energy n={n}")
+    s = nb.plummer(n, 40 + n)
+    s[3] *= np.random.default_rng(n).lognormal(0.0, 0.3, n)
+    with nb.Backend("fp64", 1e-4) as b:
+        kin, pot = b.energy_of(s)
+        kin_c, pot_c = b.energy_of(s, compat=True)
+    assert kin == pytest.approx(oracle.kinetic(s), rel=1e-13)
+    assert kin_c == kin
+    if n == 1:
+        assert pot == 0.0 and pot_c == 0.0
+        return
+    assert pot == pytest.approx(oracle.potential(s, 1e-4, nthreads=8), rel=1e-12)
+    assert pot_c == pytest.approx(oracle.potential_compat(s), rel=1e-12)
+
+
 def test_mixed_energy_drift_config2(nb):
     """N=102400 Plummer seed 2, eps = 1/256, dt = 1/1024, 3 steps: the mixed
     form's relative energy drift stays within 10x of the FP64 drift (or below
