@@ -190,15 +190,24 @@ def roofline(prec, res, traffic):
            "unit": "TFLOP/s", "frac": res["kernel_tflops"] / PEAK[prec],
            "bound_note": "vector-ALU FLOP roofline (MI355X vector peak: FP64 78.6, FP32 157.3 TFLOP/s): "
                          "the softened r^-3 pair sum is VALU work, HBM-bound by no measure and no MFMA contraction",
-           "traffic": tb, "traffic_kernel": tk,
+           "traffic": tb, "traffic_kernel": tk, "algorithmic_bytes": 104 * N,
            "kernel": kname(prec, res), "kernel_ms": res["kernel_ms"],
            "flop_per_launch": res["kernel_flop"],
            "flop_convention": "60 flop per directed pair, N^2 directed pairs per evaluation"}
     if res["shape"].get("pairing") == "symmetric":
+        out["traffic_note"] = ("mostly the FP64 atomic adds of the per-tile source totals and the target totals, "
+                               "about 3 N^2 / 512 adds per launch at ~17 B each, performed at the memory side "
+                               "(the 8 XCDs' L2s are not coherent); besides, each workgroup reads its target "
+                               "block and source blocks through L2. "
+                               "algorithmic_bytes = 7 doubles read + 6 written per body")
         out["executed"] = {"flop_per_launch": res["executed_flop"], "achieved": res["executed_tflops"],
                            "frac": res["executed_tflops"] / PEAK[prec],
                            "convention": "73 flop per unordered pair + 60 per directed pair of the "
                                          "block-diagonal pass"}
+    else:
+        out["traffic_note"] = ("the source records re-read through L2 by every workgroup, and the j-split "
+                               "partial planes written and reduced by a second kernel. "
+                               "algorithmic_bytes = 7 doubles read + 6 written per body")
     return out
 
 
