@@ -53,7 +53,6 @@ EPS = 1e-3
 DT = 1.0 / 2048
 FLOP_PER_PAIR = 60
 FLOP_PER_SYM_PAIR = 73
-SYM_BLOCK = 512   # bodies per block of the symmetric kernel
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector
 
 
@@ -109,11 +108,17 @@ def kernel_flop(shape):
     return FLOP_PER_PAIR * N * N
 
 
+def sym_block(shape):
+    """Bodies per block of the symmetric kernel: threads x targets per lane."""
+    return shape["block"] * shape["ipl"]
+
+
 def executed_flop(shape):
     """Flop the launch shape's kernels perform for one evaluation."""
     if shape.get("pairing") == "symmetric":
-        diag = N * SYM_BLOCK - N                      # directed pairs inside blocks
-        return FLOP_PER_SYM_PAIR * (N * N - N * SYM_BLOCK) // 2 + FLOP_PER_PAIR * diag
+        b = sym_block(shape)
+        inside = (N // b) * b * b + (N % b) ** 2      # ordered pairs inside blocks, self pairs included
+        return FLOP_PER_SYM_PAIR * (N * N - inside) // 2 + FLOP_PER_PAIR * (inside - N)
     return FLOP_PER_PAIR * N * N
 
 
@@ -195,10 +200,11 @@ def roofline(prec, res, traffic):
            "flop_per_launch": res["kernel_flop"],
            "flop_convention": "60 flop per directed pair, N^2 directed pairs per evaluation"}
     if res["shape"].get("pairing") == "symmetric":
-        out["traffic_note"] = ("mostly the FP64 atomic adds of the per-tile source totals and the target totals, "
-                               "about 3 N^2 / 512 adds per launch at ~17 B each, performed at the memory side "
-                               "(the 8 XCDs' L2s are not coherent); besides, each workgroup reads its target "
-                               "block and source blocks through L2. "
+        b = sym_block(res["shape"])
+        out["traffic_note"] = (f"mostly FP64 atomic adds: the per-tile source totals, about 3 N^2 / {b} per launch "
+                               f"(blocks of {b}), and the target sums, flushed every two tiles (3 N^2 / 128), "
+                               "~17 B each, performed at the memory side (the 8 XCDs' L2s are not coherent); "
+                               "besides, each workgroup reads its target block and source blocks through L2. "
                                "algorithmic_bytes = 7 doubles read + 6 written per body")
         out["executed"] = {"flop_per_launch": res["executed_flop"], "achieved": res["executed_tflops"],
                            "frac": res["executed_tflops"] / PEAK[prec],

@@ -38,16 +38,18 @@ enum nbh_status {
 enum nbh_precision {
     NBH_FP64 = 0,  /* plain FP64 pair arithmetic: the drop-in */
     NBH_MIXED = 1  /* FP64 x_j - x_i rounded to FP32; FP32 r^2, rsqrt, jerk;
-                      two-sum compensated FP32 accumulation */
+                      FP32 sums over runs of sources, carried on in an FP32
+                      two-sum (hi, lo) pair (directed pairing) or in FP64
+                      (symmetric pairing: runs of at most 192 terms) */
 };
 
 /* How pairs are evaluated.  DIRECTED: every ordered pair (i <- j) once, sums
  * in a fixed order, bit-reproducible run to run.  SYMMETRIC: every unordered
  * pair once, applied to both bodies (Newton's third law, as the reference's
  * j < i loops at src/kernel.cpp:1126,1187), sums combined with FP64 atomic
- * adds, reproducible to FP64 rounding only.  SYMMETRIC is used (in both
- * precisions) when n >= 1024; smaller n run DIRECTED (nbh_launch_pairing
- * reports which). */
+ * adds, reproducible to FP64 rounding only.  SYMMETRIC cuts the bodies into
+ * blocks of 768 and is used (in both precisions) when n >= 1536; smaller n
+ * run DIRECTED (nbh_launch_pairing reports which). */
 enum nbh_pairing {
     NBH_PAIR_DIRECTED = 0,
     NBH_PAIR_SYMMETRIC = 1

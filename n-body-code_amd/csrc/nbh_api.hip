@@ -134,8 +134,8 @@ nbh::Shape pick_shape(const nbh_ctx* c, int n)
     const nbh_config& f = c->cfg;
     if (f.pairing == NBH_PAIR_SYMMETRIC && nbh::sym_supported(f.precision, n)) {
         nbh::Shape s{};
-        s.block = nbh::sym_block(f.precision) / 2;  // threads per workgroup (2 targets per lane)
-        s.ipl = 2;
+        s.ipl = nbh::sym_targets_per_lane(f.precision);
+        s.block = nbh::sym_block(f.precision) / s.ipl;  // threads per workgroup
         s.jsplit = 1;
         s.jchunk = n;
         s.sym_chunk = nbh::sym_chunk(f.precision, n, c->cus);
@@ -225,7 +225,7 @@ extern "C" {
 
 const char* nbh_last_error(void) { return nbh::last_error(); }
 
-int nbh_version(void) { return 102; }
+int nbh_version(void) { return 103; }
 
 int nbh_device_info(int device, int* compute_units, int* wave_size, char* name, int name_len)
 {

@@ -60,7 +60,8 @@ hipError_t launch_force_blockdiag(int precision, const State& s, int n, double e
 // off-diagonal block pair once with the reaction added to the source side.
 // Usable when sym_supported(precision, n); ev0/ev1 (may be null) bracket
 // both launches.
-int sym_block(int precision);
+int sym_block(int precision);             // bodies per block
+int sym_targets_per_lane(int precision);  // block = 64 * targets per lane * waves
 bool sym_supported(int precision, int n);
 // Entries in the source list of full block P of nr (the last entry is the
 // partial block, if there is one).

@@ -35,10 +35,13 @@
 //     are summed with cross-lane shuffles, the waves' sums are combined in
 //     LDS in wave order, and the tile's 64 x 6 source totals are added to the
 //     outputs with FP64 atomics;
-//   * target sums run over the 64 sources of a tile and are added to
-//     per-lane totals (mixed: FP32 sums folded into an FP32 (hi, lo) pair
-//     with Knuth's two-sum), which go to the outputs with FP64 atomics when
-//     the workgroup's block pairs are done.
+//   * target sums run over the sources of FL consecutive tiles and are then
+//     added to the outputs with FP64 atomics (the single-system kernel, FL =
+//     2: no totals held in registers, which leaves room for three targets
+//     per lane at five waves per SIMD), or over one tile and then into
+//     per-lane totals that go out once the workgroup's block pairs are done
+//     (the ensemble's kernel, FL = 0; mixed: FP32 sums folded into an FP32
+//     (hi, lo) pair with Knuth's two-sum).
 //
 // The FP64 atomic adds make the summation order depend on scheduling: two
 // runs agree to rounding of FP64 sums (about 1e-16 relative), not bit for
@@ -299,8 +302,9 @@ struct SymArgs {
 };
 
 // Full block P of one system (bodies 0..n-1 of s and out) against entries
-// i0 .. i1 of its source list: the body of both symmetric kernels.
-template <template <int> class TT, int K, int W>
+// i0 .. i1 of its source list: the body of both symmetric kernels.  FL > 0:
+// target sums go to the outputs every FL tiles; FL = 0: into per-lane totals.
+template <template <int> class TT, int K, int W, int FL>
 __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& out, int n, double eps2d, int P,
                                                 int i0, int i1)
 {
@@ -341,16 +345,16 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
     }
     __syncthreads();
 
-    typename T::Tot tot;
-    T::zero(tot);
+    typename T::Tot tot;  // unused (and optimised away) when FL > 0
+    if constexpr (FL == 0) T::zero(tot);
+    Sum ai[K][6];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ai[k][c] = Sum(0);
     for (int tt = 0; tt < ntiles; ++tt) {
         const int buf = tt & 1;
         const bool more = tt + 1 < ntiles;
-        Sum ai[K][6];
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) ai[k][c] = Sum(0);
 #pragma unroll 1
         for (int g = 0; g < 4; ++g) {
             Sum aj[6] = {Sum(0), Sum(0), Sum(0), Sum(0), Sum(0), Sum(0)};
@@ -366,7 +370,21 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
                 part[buf][w][g * 16 + p][c] = v;
             }
         }
-        T::fold(tot, ai);
+        if constexpr (FL == 0) {
+            T::fold(tot, ai);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) ai[k][c] = Sum(0);
+        } else if (tt % FL == FL - 1 || !more) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    unsafeAtomicAdd(&out.c[c][ibase + k * 64], (double)ai[k][c]);
+                    ai[k][c] = Sum(0);
+                }
+        }
         // the next tile is loaded after the arithmetic, not held in registers
         // across it (12 VGPRs); the other resident workgroups cover the load
         if (more && t < kSymTile) {
@@ -388,18 +406,22 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
             }
         }
     }
+    if constexpr (FL == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+        for (int k = 0; k < K; ++k)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) unsafeAtomicAdd(&out.c[c][ibase + k * 64], T::total(tot, k, c));
+            for (int c = 0; c < 6; ++c) unsafeAtomicAdd(&out.c[c][ibase + k * 64], T::total(tot, k, c));
+    }
 }
 
 // Resident waves per SIMD asked of the compiler: T::kWaves (mixed 5: 96
-// VGPRs, a few spills; FP64 3), or fewer where the workgroups' LDS (160 KiB
-// per CU) holds fewer (mixed with 2 waves: 18 KiB, 8 workgroups, 4 waves).
-// gfx950 issues FP32 at twice the one-wave rate only by pairing instructions
-// of two waves, so more resident waves raise the dual-issue share (mixed:
-// +1.6% at n = 2^20 over 4 waves).
+// VGPRs with three targets per lane, its few spills outside the step loop;
+// FP64 3: 166 VGPRs), or fewer where the workgroups' LDS (160 KiB per CU)
+// holds fewer (mixed with 2 waves: 18 KiB, 8 workgroups, 4 waves).  gfx950
+// issues FP32 at twice the one-wave rate only by pairing instructions of two
+// waves, so more resident waves raise the dual-issue share (mixed, two
+// targets per lane with two-sum totals: +1.6% at n = 2^20 over 4 waves;
+// three targets per lane with flushed sums: 5 waves 5% faster than 4).
 template <class T, int W>
 constexpr int sym_waves()
 {
@@ -407,6 +429,13 @@ constexpr int sym_waves()
     constexpr int by_lds = (160 * 1024 / lds) * W / 4;
     return by_lds < T::kWaves ? by_lds : T::kWaves;
 }
+
+// Tiles per flush of the target sums in the single-system kernel: FP32 runs
+// of 128 sources.  Measured at n = 2^20 (three targets per lane, five waves):
+// one tile per flush ran no faster than two targets per lane with two-sum
+// totals in registers, two tiles 3.5% faster, four 5% faster but with twice
+// the largest acceleration error.
+constexpr int kSysFlush = 2;
 
 template <template <int> class TT, int K, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_waves<TT<K>, W>(), 8))) void k_force_sym(
@@ -418,7 +447,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_wave
     const int i0 = blockIdx.y * a.chunk;
     if (i0 >= len) return;  // uniform over the workgroup, before any barrier
     const int i1 = min(len, i0 + a.chunk) - 1;
-    sym_block_pairs<TT, K, W>(a.s, a.out, a.n, a.eps2, P, i0, i1);
+    sym_block_pairs<TT, K, W, kSysFlush>(a.s, a.out, a.n, a.eps2, P, i0, i1);
 }
 
 // Many systems: one workgroup per item of a list (system offset and size,
@@ -434,22 +463,25 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_wave
     Forces o = out;
 #pragma unroll
     for (int c = 0; c < 6; ++c) o.c[c] += it.off;
-    sym_block_pairs<TT, K, W>(q, o, it.n, it.eps2, it.P, it.i0, it.i1);
+    sym_block_pairs<TT, K, W, 0>(q, o, it.n, it.eps2, it.P, it.i0, it.i1);
 }
 
-// Launch shape per precision: targets per lane, waves per workgroup (one
-// system; the ensemble's, whose diagonal blocks are a larger share of the
-// pairs).
-constexpr int kMixK = 2, kMixW = 4, kMixEnsW = 2;
-constexpr int kF64K = 2, kF64W = 4, kF64EnsW = 4;
+// Launch shape per precision: targets per lane and waves per workgroup of
+// one system's kernel (blocks of 768) and of the ensemble's (targets in
+// registers: blocks of 512 in FP64, 256 mixed, since its systems are small
+// and their diagonal blocks a larger share of the pairs).
+constexpr int kMixK = 3, kMixW = 4, kMixEnsK = 2, kMixEnsW = 2;
+constexpr int kF64K = 3, kF64W = 4, kF64EnsK = 2, kF64EnsW = 4;
 
 }  // namespace
 
 int sym_block(int precision) { return precision == 0 ? 64 * kF64K * kF64W : 64 * kMixK * kMixW; }
 
+int sym_targets_per_lane(int precision) { return precision == 0 ? kF64K : kMixK; }
+
 bool sym_supported(int precision, int n) { return n >= 2 * sym_block(precision); }
 
-int ens_sym_block(int precision) { return precision == 0 ? 64 * kF64K * kF64EnsW : 64 * kMixK * kMixEnsW; }
+int ens_sym_block(int precision) { return precision == 0 ? 64 * kF64EnsK * kF64EnsW : 64 * kMixEnsK * kMixEnsW; }
 
 bool ens_sym_supported(int precision, int n) { return n >= 2 * ens_sym_block(precision); }
 
@@ -484,9 +516,9 @@ hipError_t launch_ens_sym(int precision, const State& s, const Forces& out, cons
 {
     if (nitems <= 0) return hipSuccess;
     if (precision == 0)
-        k_ens_sym<SymF64, kF64K, kF64EnsW><<<nitems, 64 * kF64EnsW, 0, st>>>(s, out, items);
+        k_ens_sym<SymF64, kF64EnsK, kF64EnsW><<<nitems, 64 * kF64EnsW, 0, st>>>(s, out, items);
     else
-        k_ens_sym<SymMixed, kMixK, kMixEnsW><<<nitems, 64 * kMixEnsW, 0, st>>>(s, out, items);
+        k_ens_sym<SymMixed, kMixEnsK, kMixEnsW><<<nitems, 64 * kMixEnsW, 0, st>>>(s, out, items);
     return hipGetLastError();
 }
 

@@ -25,13 +25,25 @@ def force(nb, state, eps2, pairing, precision="mixed"):
         return b.force(state), shape
 
 
-# n = 1024 .. 2560: 2, 3, 4, 5 blocks of 512 (even and odd ring); 1025, 1100,
-# 1224, 1324, 2500, 3001: a partial last block (1, 76, 200, 300, 452, 441
-# bodies), a source block only, of which 1, 2, 4, 5, 8, 7 of 8 tiles are
-# visited, the last one padded inside
+def block_of(nb, precision):
+    """Bodies per block of the symmetric kernel (threads x targets per lane)."""
+    with nb.Backend(precision, 1e-4, pairing="symmetric") as b:
+        shape = b.launch_shape(1 << 16)
+    assert shape["pairing"] == "symmetric" and shape["block"] * shape["ipl"] % 64 == 0
+    return shape["block"] * shape["ipl"]
+
+
+# 2, 3, 4, 5 full blocks of B (even and odd ring); then a partial last block of
+# 1, 76, 200, 300 bodies after 2 full blocks and 452, 441 after 3: a source
+# block only, of which its first 1, 2, 4, 5, 8, 7 tiles of 64 are visited, the
+# last one padded inside
+SIZES = [(2, 0), (3, 0), (4, 0), (5, 0), (2, 1), (2, 76), (2, 200), (2, 300), (3, 452), (3, 441)]
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-@pytest.mark.parametrize("n", [1024, 1536, 2048, 2560, 1025, 1100, 1224, 1324, 2500, 3001])
-def test_symmetric_matches_oracle(nb, oracle, n, precision):
+@pytest.mark.parametrize("blocks,rest", SIZES)
+def test_symmetric_matches_oracle(nb, oracle, blocks, rest, precision):
+    n = blocks * block_of(nb, precision) + rest
     _banner(f"oracle {precision} n={n}")
     s = nb.plummer(n, 11 + n)
     eps2 = (1 / 64) ** 2
@@ -66,11 +78,12 @@ def test_symmetric_close_to_directed(nb, precision):
         assert ea.max() <= 1e-12 and ej.max() <= 1e-12
 
 
-# 399 full blocks and a partial one of 100 bodies: workgroups take up to 5
+# 399 full blocks and a partial one of 100 bodies: workgroups take several
 # entries of a block's source list, which ends with the partial block
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_partial_block_inside_workgroup_ranges(nb, oracle, precision):
-    n = 399 * 512 + 100
+    B = block_of(nb, precision)
+    n = 399 * B + 100
     _banner(f"partial block, multi-pair workgroups {precision} n={n}")
     s = nb.plummer(n, 23)
     eps2 = (1 / 256) ** 2
@@ -82,8 +95,8 @@ def test_partial_block_inside_workgroup_ranges(nb, oracle, precision):
     _banner(f"{precision} vs directed: acc median {np.median(ea):.2e} max {ea.max():.2e}; "
             f"jerk median {np.median(ej):.2e} max {ej.max():.2e}")
     # a sample from every block kind: the partial one, its neighbours, the first
-    idx = np.unique(np.concatenate([np.arange(n - 100, n), np.arange(n - 612, n - 100, 8),
-                                    np.arange(0, 512, 8), np.random.default_rng(9).choice(n, 64)]))
+    idx = np.unique(np.concatenate([np.arange(n - 100, n), np.arange(n - 100 - B, n - 100, 8),
+                                    np.arange(0, B, 8), np.random.default_rng(9).choice(n, 64)]))
     ref = oracle.force_sample(s, eps2, idx, nthreads=16)
     eo = body_relerr(fs[:3, idx], ref[:3])
     eoj = body_relerr(fs[3:, idx], ref[3:])
@@ -111,10 +124,11 @@ def test_third_law_momentum(nb, precision, rel):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-@pytest.mark.parametrize("n", [2048, 2100])
-def test_eps2_zero_diagonal_pass(nb, oracle, precision, n):
+@pytest.mark.parametrize("rest", [0, 52])
+def test_eps2_zero_diagonal_pass(nb, oracle, precision, rest):
     """eps2 = 0: the self pair is dropped in the block-diagonal pass, and the
     pad bodies of a partial block stay away from every body."""
+    n = 3 * block_of(nb, precision) + rest
     s = nb.plummer(n, 13)
     f, _ = force(nb, s, 0.0, "symmetric", precision)
     assert np.isfinite(f).all()
@@ -124,7 +138,8 @@ def test_eps2_zero_diagonal_pass(nb, oracle, precision, n):
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_unsupported_n_runs_directed(nb, precision):
-    for n in (1000, 512, 1023, 7):
+    B = block_of(nb, precision)
+    for n in (2 * B - 1, B, 1000, 7):
         s = nb.plummer(n, 2)
         fs, shape = force(nb, s, 1e-4, "symmetric", precision)
         assert shape["pairing"] == "directed"
