@@ -54,6 +54,7 @@ DT = 1.0 / 2048
 FLOP_PER_PAIR = 60
 FLOP_PER_SYM_PAIR = 73
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector
+SYM_FLUSH_TILES = {"fp64": 8, "mixed": 2}  # tiles per flush of the symmetric kernels' target sums
 
 
 def body_relerr(a, b):
@@ -201,11 +202,12 @@ def roofline(prec, res, traffic):
            "flop_convention": "60 flop per directed pair, N^2 directed pairs per evaluation"}
     if res["shape"].get("pairing") == "symmetric":
         b = sym_block(res["shape"])
+        fl = SYM_FLUSH_TILES[prec]
         out["traffic_note"] = (f"mostly FP64 atomic adds: the per-tile source totals, about 3 N^2 / {b} per launch "
-                               f"(blocks of {b}), and the target sums, flushed every two tiles (3 N^2 / 128), "
-                               "~17 B each, performed at the memory side (the 8 XCDs' L2s are not coherent); "
-                               "besides, each workgroup reads its target block and source blocks through L2. "
-                               "algorithmic_bytes = 7 doubles read + 6 written per body")
+                               f"(blocks of {b}), and the target sums, flushed every {fl} tiles of 64 sources "
+                               f"(3 N^2 / {64 * fl}), performed at the memory side (the 8 XCDs' L2s are not "
+                               "coherent); besides, each workgroup reads its target block and source blocks "
+                               "through L2. algorithmic_bytes = 7 doubles read + 6 written per body")
         out["executed"] = {"flop_per_launch": res["executed_flop"], "achieved": res["executed_tflops"],
                            "frac": res["executed_tflops"] / PEAK[prec],
                            "convention": "73 flop per unordered pair + 60 per directed pair of the "

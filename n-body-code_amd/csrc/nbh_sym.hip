@@ -37,8 +37,8 @@
 //     outputs with FP64 atomics;
 //   * target sums run over the sources of FL consecutive tiles and are then
 //     added to the outputs with FP64 atomics (the single-system kernel, FL =
-//     2: no totals held in registers, which leaves room for three targets
-//     per lane at five waves per SIMD), or over one tile and then into
+//     2 mixed, 8 FP64: no totals held in registers, which leaves room for
+//     three targets per lane at five waves per SIMD in mixed), or over one tile and then into
 //     per-lane totals that go out once the workgroup's block pairs are done
 //     (the ensemble's kernel, FL = 0; mixed: FP32 sums folded into an FP32
 //     (hi, lo) pair with Knuth's two-sum).
@@ -94,6 +94,12 @@ __device__ __forceinline__ double row_rot(double v)
 template <int K>
 struct SymMixed {
     static constexpr int kWaves = 5;
+    // tiles per flush of the target sums in the single-system kernel: FP32
+    // runs of 128 sources.  Measured at n = 2^20 (three targets per lane,
+    // five waves): one tile per flush ran no faster than two targets per lane
+    // with two-sum totals in registers, two tiles 3.5% faster, four 5% faster
+    // but with twice the largest acceleration error
+    static constexpr int kFlushTiles = 2;
     using Rec = JBMX;
     using Sum = float;
     using Eps = float;
@@ -196,6 +202,9 @@ struct __align__(16) JB80 {
 template <int K>
 struct SymF64 {
     static constexpr int kWaves = 3;
+    // FP64 runs lose nothing by being long: 8 tiles per flush (512 sources)
+    // were 2% faster than 2 at n = 2^20
+    static constexpr int kFlushTiles = 8;
     using Rec = JB80;
     using Sum = double;
     using Eps = double;
@@ -430,13 +439,6 @@ constexpr int sym_waves()
     return by_lds < T::kWaves ? by_lds : T::kWaves;
 }
 
-// Tiles per flush of the target sums in the single-system kernel: FP32 runs
-// of 128 sources.  Measured at n = 2^20 (three targets per lane, five waves):
-// one tile per flush ran no faster than two targets per lane with two-sum
-// totals in registers, two tiles 3.5% faster, four 5% faster but with twice
-// the largest acceleration error.
-constexpr int kSysFlush = 2;
-
 template <template <int> class TT, int K, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_waves<TT<K>, W>(), 8))) void k_force_sym(
     SymArgs a)
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_wave
     const int i0 = blockIdx.y * a.chunk;
     if (i0 >= len) return;  // uniform over the workgroup, before any barrier
     const int i1 = min(len, i0 + a.chunk) - 1;
-    sym_block_pairs<TT, K, W, kSysFlush>(a.s, a.out, a.n, a.eps2, P, i0, i1);
+    sym_block_pairs<TT, K, W, TT<K>::kFlushTiles>(a.s, a.out, a.n, a.eps2, P, i0, i1);
 }
 
 // Many systems: one workgroup per item of a list (system offset and size,
