@@ -497,10 +497,10 @@ int sym_chunk(int precision, int n, int cus)
 {
     const int b = sym_block(precision);
     const int per = sym_list_max(precision, n);
-    // about 64 workgroups per CU in all (some 16 rounds of resident ones),
+    // about 128 workgroups per CU in all (some 25 rounds of resident ones),
     // so the last round is a small part; measured at n = 2^18 and 2^20
-    // against 8..256: within 1-3% of the best
-    const long want = 64L * (cus > 0 ? cus : 256);
+    // against 16..256 per CU: within 0.5% of the best (64: 1% slower at 2^18)
+    const long want = 128L * (cus > 0 ? cus : 256);
     long c = ((long)(n / b) * per + want - 1) / want;
     if (c < 1) c = 1;
     if (c > per) c = per;
