@@ -187,10 +187,12 @@ int nbh_timing(nbh_ctx *ctx, double *force_ms, int64_t *force_launches,
                double *step_ms, int64_t *steps);
 int nbh_reset_timing(nbh_ctx *ctx);
 
-/* Launch shape actually used for n bodies: block, jsplit, workgroups. */
+/* Launch shape actually used for n bodies: block (threads per workgroup),
+ * jsplit, workgroups.  For the symmetric kernel, block x targets per lane is
+ * its block of bodies and workgroups includes the block-diagonal pass. */
 int nbh_launch_shape(nbh_ctx *ctx, int n, int *block, int *jsplit,
                      int *workgroups);
-/* Targets per lane of that shape. */
+/* Targets per lane of that shape (directed: 1 or 2; symmetric: 3). */
 int nbh_launch_ipl(nbh_ctx *ctx, int n, int *ipl);
 /* Pairing (enum nbh_pairing) that a force evaluation of n bodies uses. */
 int nbh_launch_pairing(nbh_ctx *ctx, int n, int *pairing);
