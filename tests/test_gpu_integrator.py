@@ -192,6 +192,28 @@ def test_energy_ragged_sizes(nb, oracle, n):
     assert pot_c == pytest.approx(oracle.potential_compat(s), rel=1e-12)
 
 
+def test_energy_million_bodies_properties(nb):
+    """N = 2^20 (4096 tiles, an even ring): doubling every mass scales the
+    kinetic energy by 2 and both potentials by 4 exactly (powers of two), and
+    a permutation of the bodies moves the sums only by rounding.  A tile pair
+    visited twice or missed would not survive the permutation."""
+    n = 1 << 20
+    s = nb.plummer(n, 3)
+    perm = np.random.default_rng(5).permutation(n)
+    s2 = s.copy()
+    s2[3] *= 2.0
+    with nb.Backend("fp64", 1e-6) as b:
+        kin, pot = b.energy_of(s)
+        kin2, pot2 = b.energy_of(s2)
+        kinp, potp = b.energy_of(s[:, perm])
+        _, potc = b.energy_of(s, compat=True)
+        _, potc2 = b.energy_of(s2, compat=True)
+    _banner(f"N=2^20 energy: kin {kin:.15e} pot {pot:.15e}; permuted pot rel diff {abs(potp - pot) / abs(pot):.1e}")
+    assert kin2 == 2.0 * kin and pot2 == 4.0 * pot and potc2 == 4.0 * potc
+    assert abs(kinp - kin) <= 1e-13 * kin
+    assert abs(potp - pot) <= 1e-12 * abs(pot)
+
+
 def test_mixed_energy_drift_config2(nb):
     """N=102400 Plummer seed 2, eps = 1/256, dt = 1/1024, 3 steps: the mixed
     form's relative energy drift stays within 10x of the FP64 drift (or below
