@@ -203,3 +203,35 @@ def test_resident_steps_and_graph(nb, precision, n):
     assert np.abs(xs_e - xs_g).max() <= 1e-10
     assert dx <= (1e-6 if precision == "mixed" else 1e-11)
     assert ds_e <= max(10 * dd, 1e-10)
+
+
+def test_million_bodies_symmetric_vs_directed(nb, oracle):
+    """N = 2^20 (1365 full blocks of 768 and a partial one of 256 bodies): FP64
+    symmetric per body to 1e-12 against the oracle's sum on a sample (the
+    partial block, its neighbour and random bodies) and to 2e-12 against the
+    FP64 directed kernel (two sums in different orders, each within 1e-12),
+    mixed symmetric against FP64 to the mixed bar, and the third law's
+    momentum sum."""
+    n = 1 << 20
+    s = nb.plummer(n, 3)
+    eps2 = 1e-6
+    fd, _ = force(nb, s, eps2, "directed", "fp64")
+    fs, shape = force(nb, s, eps2, "symmetric", "fp64")
+    assert shape["pairing"] == "symmetric"
+    ea = body_relerr(fs[:3], fd[:3])
+    ej = body_relerr(fs[3:], fd[3:])
+    idx = np.unique(np.concatenate([np.arange(n - 8, n), np.arange(n - 300, n - 292),
+                                    np.random.default_rng(1).choice(n, 16, replace=False)]))
+    ref = oracle.force_sample(s, eps2, idx, nthreads=16)
+    eo = body_relerr(fs[:3, idx], ref[:3])
+    eoj = body_relerr(fs[3:, idx], ref[3:])
+    fm, _ = force(nb, s, eps2, "symmetric", "mixed")
+    em = body_relerr(fm[:3], fd[:3])
+    _banner(f"N=2^20 fp64 sym vs oracle sample acc {eo.max():.2e} jerk {eoj.max():.2e}; vs dir acc max "
+            f"{ea.max():.2e} jerk max {ej.max():.2e}; mixed sym acc median {np.median(em):.2e} max {em.max():.2e}")
+    assert eo.max() <= 1e-12 and eoj.max() <= 1e-12
+    assert ea.max() <= 2e-12 and ej.max() <= 2e-12
+    assert np.median(em) <= 1e-6 and em.max() <= 1e-5
+    m = s[3]
+    for c in range(3):
+        assert abs(np.sum(m * fs[c])) <= 1e-13 * np.sum(np.abs(m * fs[c]))
