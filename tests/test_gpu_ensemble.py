@@ -242,3 +242,42 @@ def test_symmetric_ensemble_matches_directed_and_graph(nb, symset, precision):
     _banner(f"{precision}: 16 steps, max |x_sym - x_dir| {d:.2e}, |graph - eager| {np.abs(xs - xg).max():.2e}")
     assert d <= (1e-12 if precision == "fp64" else 1e-6)
     assert np.abs(xs - xg).max() <= (1e-13 if precision == "fp64" else 1e-9)
+
+
+@pytest.mark.parametrize("precision,pairing", [("fp64", "directed"), ("fp64", "symmetric"), ("mixed", "symmetric")])
+def test_full_cluster_set(nb, oracle, precision, pairing):
+    """The 8192-cluster workload itself (n_k log-uniform in [8, 4096], seed 4,
+    eps_k = 1/(4 n_k^(1/3))): the momentum sum of every system vanishes to
+    rounding, and a sample of systems (the six largest, which the symmetric
+    kernel covers, the smallest, random ones) matches the oracle to the
+    precision's bar."""
+    counts = nb.ensemble_sizes(8192, 8, 4096, 4)
+    eps2 = (1.0 / (4.0 * counts.astype(float) ** (1.0 / 3.0))) ** 2
+    state = nb.ensemble_state(counts, 4)
+    _banner(f"full cluster set {precision} {pairing}: {len(counts)} systems, {counts.sum()} bodies")
+    with nb.Backend(precision, pairing=pairing) as b, nb.Ensemble(b, counts, eps2) as e:
+        e.upload(state)
+        _, f = e.download()
+        rel = []
+        for k in range(len(counts)):
+            s, fk = e.system(state, k), e.system(f, k)
+            m = s[3]
+            for c in range(3):
+                rel.append(abs(np.sum(m * fk[c])) / max(np.sum(np.abs(m * fk[c])), 1e-300))
+        pick = np.unique(np.concatenate([np.argsort(counts)[-6:], [np.argmin(counts)],
+                                         np.random.default_rng(8).choice(len(counts), 9, replace=False)]))
+        worst, meds = 0.0, []
+        for k in pick:
+            s, fk = e.system(state, k), e.system(f, k)
+            ref = oracle.force(s, eps2[k], nthreads=8)
+            ea = body_relerr(fk[:3], ref[:3])
+            ej = body_relerr(fk[3:], ref[3:])
+            worst = max(worst, ea.max(), ej.max())
+            meds.append(np.median(ea))
+    _banner(f"  momentum sum worst {max(rel):.1e}; sample of {len(pick)} systems (largest {counts.max()}): "
+            f"worst {worst:.2e}, median of medians {np.median(meds):.2e}")
+    assert max(rel) <= (1e-12 if precision == "fp64" else 1e-5)
+    if precision == "fp64":
+        assert worst <= 1e-12
+    else:
+        assert np.median(meds) <= 1e-6 and max(meds) <= 1e-5
