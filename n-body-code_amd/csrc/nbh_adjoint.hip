@@ -216,7 +216,10 @@ hipError_t launch_force_vjp(const State& s, int n, double eps2, const double* co
     const int S = (n + a.jchunk - 1) / a.jchunk;
     a.eps2 = eps2;
     const dim3 grid(tiles, S);
-    if (eps2 == 0.0)
+    // the self pair is dropped explicitly when eps2 is small enough for its
+    // r^7 = eps2^-7/2 term to overflow FP64 (eps2 < ~1e-88); above, it adds
+    // finite multiples of d = u = 0
+    if (eps2 < 1e-80)
         k_force_vjp<true><<<grid, kVB, 0, st>>>(a);
     else
         k_force_vjp<false><<<grid, kVB, 0, st>>>(a);
@@ -287,7 +290,7 @@ int adj_force(nbh_adjoint* a, double* st, double* f)
     sh.jchunk = per * 64;
     sh.jsplit = (n + sh.jchunk - 1) / sh.jchunk;
     hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(st, n), n, c->cfg.eps2, sh, nbh::forces_of(f, n),
-                                     a->partial.d(), nullptr, c->stream, nullptr, nullptr);  // LDS tiles
+                                     a->partial.d(), nullptr, nullptr, c->stream, nullptr, nullptr);  // LDS tiles
     if (e != hipSuccess) return fail(NBH_ERR_HIP, "adjoint force launch failed: %s", hipGetErrorString(e));
     return NBH_OK;
 }

@@ -210,7 +210,7 @@ int enqueue_force(nbh_ctx* c, const nbh::State& s, int n, const nbh::Forces& out
         int rc = get_events(c, &ep);
         if (rc) return rc;
     }
-    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, out, part, c->rec.p, st,
+    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, out, part, c->rec.p, c->vwork.d(), st,
                                      timed ? ep.a : nullptr, timed ? ep.b : nullptr);
     if (timed) (e == hipSuccess ? c->ev_pending : c->ev_free).push_back(ep);
     if (e != hipSuccess) return fail(NBH_ERR_HIP, "force launch failed: %s", hipGetErrorString(e));
@@ -269,6 +269,7 @@ int nbh_create(const nbh_config* cfg, nbh_ctx** out)
     }
     e = c->scal.reserve(sizeof(double) * 8);
     if (e == hipSuccess) e = c->flag.reserve(sizeof(int) * 2);
+    if (e == hipSuccess) e = c->vwork.reserve(sizeof(double) * nbh::vref_work_doubles());
     if (e != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -290,7 +291,7 @@ int nbh_destroy(nbh_ctx* c)
         (void)hipEventDestroy(ep.b);
     }
     for (DevBuf* b : {&c->in, &c->out, &c->partial, &c->rec, &c->flag, &c->scal, &c->state, &c->f0,
-                      &c->f1, &c->ework})
+                      &c->f1, &c->ework, &c->vwork})
         b->release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -388,7 +389,7 @@ int nbh_autotune(nbh_ctx* c, int n, int* block_out, int* ipl_out, int* jsplit_ou
             hipError_t e = hipEventRecord(e0, c->stream);
             if (e == hipSuccess)
                 e = nbh::launch_force(c->cfg.precision, st, n, c->cfg.eps2, sh, fo, c->partial.d(), c->rec.p,
-                                      c->stream, nullptr, nullptr, trial_nj);
+                                      c->vwork.d(), c->stream, nullptr, nullptr, trial_nj);
             if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;
@@ -505,8 +506,8 @@ int nbh_force(nbh_ctx* c, int n, const double* x, const double* y, const double*
     if (rc) return rc;
     int flag = 0;
     float ms = 0.f;
-    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, f, part, c->rec.p, c->stream,
-                                     ep.a, ep.b);
+    hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, f, part, c->rec.p, c->vwork.d(),
+                                     c->stream, ep.a, ep.b);
     const char* what = "force launch";
     if (e == hipSuccess) {
         what = "finite check";

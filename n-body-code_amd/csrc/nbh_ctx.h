@@ -61,7 +61,8 @@ struct nbh_ctx {
     nbh::DevBuf in;       // 7n
     nbh::DevBuf out;      // 6n
     nbh::DevBuf partial;  // 6 * jsplit * n
-    nbh::DevBuf rec;      // mixed: packed source records, 48 B per body
+    nbh::DevBuf rec;      // packed source records: 64 B (FP64) / 48 B (mixed) per body
+    nbh::DevBuf vwork;    // mixed: reference-velocity reduction, vref_work_doubles()
     nbh::DevBuf flag;     // int
     nbh::DevBuf scal;     // scalars: [0] dt, [1] dt_next bits, [2..3] energy, [4..] spare
 

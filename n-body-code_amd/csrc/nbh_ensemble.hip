@@ -65,6 +65,7 @@ struct EnsArgs {
     int nitems;
     unsigned* counter;
     double* partial;  // [slot][6][kMaxTargets]
+    const double* vref;  // mixed: reference velocity of each system, [3 * nsys]
 };
 
 int bucket_of(int n)
@@ -101,11 +102,14 @@ __device__ __forceinline__ void ens_item(const EnsArgs& a, const EnsItem& it, JB
     }
     double out[IPL][6];
     if constexpr (MIXED) {
+        const VRef vr = load_vref(a.vref + 3 * it.sys);
         float vxi[IPL], vyi[IPL], vzi[IPL];
 #pragma unroll
         for (int q = 0; q < IPL; ++q) {
             const int ic = q * kLanes + lane < it.icount ? ti[q] : it.i0;
-            vxi[q] = (float)a.s.vx[ic]; vyi[q] = (float)a.s.vy[ic]; vzi[q] = (float)a.s.vz[ic];
+            vxi[q] = (float)(a.s.vx[ic] - vr.x);
+            vyi[q] = (float)(a.s.vy[ic] - vr.y);
+            vzi[q] = (float)(a.s.vz[ic] - vr.z);
         }
         const float e2 = (float)eps2;
         float hi[IPL][6], lo[IPL][6];
@@ -116,7 +120,7 @@ __device__ __forceinline__ void ens_item(const EnsArgs& a, const EnsItem& it, JB
         for (int jt = it.j0; jt < it.j1; jt += it.tile) {
             const int cnt = min(it.tile, it.j1 - jt);
             __syncthreads();
-            for (int k = lane; k < cnt; k += kLanes) tile[k] = load_jbmx(a.s, jt + k);
+            for (int k = lane; k < cnt; k += kLanes) tile[k] = load_jbmx(a.s, jt + k, vr);
             __syncthreads();
             for (int k0 = 0; k0 < cnt; k0 += kFold) {
                 float acc[IPL][6];
@@ -240,6 +244,37 @@ __global__ void k_zero_u32(unsigned* p)
     if (threadIdx.x == 0) *p = 0u;
 }
 
+// Centre-of-mass velocity of every system (mixed precision's reference
+// velocity, subtracted before velocities are rounded to FP32): one wave per
+// system, lane sums combined by a fixed butterfly (deterministic); 0 for a
+// system without mass.
+__global__ __launch_bounds__(kLanes) void k_ens_vref(State s, const EnsSys* __restrict__ sys, int nsys,
+                                                   double* __restrict__ vref)
+{
+    const int k = blockIdx.x;
+    if (k >= nsys) return;
+    const EnsSys q = sys[k];
+    double m = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    for (int i = q.off + (int)threadIdx.x; i < q.off + q.n; i += kLanes) {
+        const double mi = s.m[i];
+        m += mi;
+        px = fma(mi, s.vx[i], px);
+        py = fma(mi, s.vy[i], py);
+        pz = fma(mi, s.vz[i], pz);
+    }
+    for (int off = kLanes / 2; off > 0; off >>= 1) {
+        m += __shfl_xor(m, off, kLanes);
+        px += __shfl_xor(px, off, kLanes);
+        py += __shfl_xor(py, off, kLanes);
+        pz += __shfl_xor(pz, off, kLanes);
+    }
+    if (threadIdx.x == 0) {
+        vref[3 * k + 0] = m > 0.0 ? px / m : 0.0;
+        vref[3 * k + 1] = m > 0.0 ? py / m : 0.0;
+        vref[3 * k + 2] = m > 0.0 ? pz / m : 0.0;
+    }
+}
+
 // Per-system kinetic and softened potential energy (FP64), one workgroup per
 // system, block sums in a fixed order.
 constexpr int kEnBlock = 256;
@@ -320,7 +355,8 @@ struct nbh_ensemble {
     int total = 0;
     std::vector<int> off, cnt;
     std::vector<double> eps2;
-    bool any_zero_eps = false;
+    // some system's eps2 needs the self pair dropped (self_mask), per precision
+    bool any_mask[2] = {false, false};
     int tile[NBH_ENS_BUCKETS] = {128, 128, 128, 128};
     int split[NBH_ENS_BUCKETS] = {1, 1, 1, 1};
     int ipl = 1;  // targets per lane, every work item
@@ -328,7 +364,7 @@ struct nbh_ensemble {
     bool sym = false;  // systems of >= 2 symmetric blocks use symmetric pairing
 
     // device
-    DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot, symitems;
+    DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot, symitems, vref;
     int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0, nsym = 0;
     int list_ipl = 1;  // targets per lane of the uploaded work list
     bool uploaded = false, dt_set = false;
@@ -378,7 +414,7 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
                 for (int ib = b0; ib < b1; ib += per)
                     items.push_back({k, off + ib, std::min(per, b1 - ib), off + b0, off + b1, T, -1, Q});
                 const int len = P < nr ? nbh::sym_list_len(nr, P, nb > nr) : 0;
-                if (len >= 1) sym.push_back({off, n, P, 0, len - 1, e->eps2[k]});
+                if (len >= 1) sym.push_back({off, n, P, 0, len - 1, k, e->eps2[k]});
             }
             continue;
         }
@@ -440,7 +476,8 @@ int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::
     e->nslots = nslots;
     e->maxtile = maxtile;
     e->list_ipl = ipl;
-    const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, maxtile, e->any_zero_eps, ipl);
+    const void* k =
+        nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, maxtile, e->any_mask[c->cfg.precision != 0], ipl);
     if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d, %d targets per lane", maxtile, ipl);
     int per_cu = 0;
     NBH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, nbh::kLanes, 0));
@@ -473,8 +510,14 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
     a.nitems = e->nitems;
     a.counter = static_cast<unsigned*>(e->counter.p);
     a.partial = e->partial.d();
+    a.vref = e->vref.d();
     nbh::k_zero_u32<<<1, 64, 0, c->stream>>>(a.counter);
     NBH_HIP(hipGetLastError());
+    const bool mixed = c->cfg.precision == NBH_MIXED;
+    if (mixed) {
+        nbh::k_ens_vref<<<e->nsys, nbh::kLanes, 0, c->stream>>>(a.s, a.sys, e->nsys, e->vref.d());
+        NBH_HIP(hipGetLastError());
+    }
     nbh::EventGuard ev;
     const bool timed = c->profiling && allow_events;
     if (timed) {
@@ -482,7 +525,7 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
         if (rc) return rc;
         NBH_HIP(hipEventRecord(ev.ep.a, c->stream));
     }
-    const void* k = nbh::pick_ens_kernel(c->cfg.precision == NBH_MIXED, e->maxtile, e->any_zero_eps, e->list_ipl);
+    const void* k = nbh::pick_ens_kernel(mixed, e->maxtile, e->any_mask[mixed], e->list_ipl);
     if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d", e->maxtile);
     void* args[] = {&a};
     NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, c->stream));
@@ -497,7 +540,7 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
     }
     if (e->nsym > 0) {
         NBH_HIP(nbh::launch_ens_sym(c->cfg.precision, a.s, out, static_cast<const nbh::SymItem*>(e->symitems.p),
-                                    e->nsym, c->stream));
+                                    e->nsym, mixed ? a.vref : nullptr, c->stream));
         if (timed) {
             NBH_HIP(hipEventRecord(ev.ep.b, c->stream));
             ev.pending();
@@ -557,12 +600,13 @@ int nbh_ens_create(nbh_ctx* ctx, int nsys, const int* counts, const double* eps2
         e->off[k] = o;
         sys[k] = {o, counts[k], eps2[k]};
         o += counts[k];
-        if (eps2[k] == 0.0) e->any_zero_eps = true;
+        for (int p = 0; p < 2; ++p) e->any_mask[p] |= nbh::self_mask(p, eps2[k]);
     }
     const size_t nb = sizeof(double) * (size_t)total;
     if (e->sys.reserve(sizeof(nbh::EnsSys) * nsys) != hipSuccess || e->state.reserve(7 * nb) != hipSuccess ||
         e->f0.reserve(6 * nb) != hipSuccess || e->f1.reserve(6 * nb) != hipSuccess ||
         e->counter.reserve(sizeof(unsigned)) != hipSuccess || e->dt.reserve(sizeof(double)) != hipSuccess ||
+        e->vref.reserve(sizeof(double) * 3 * (size_t)nsys) != hipSuccess ||
         e->ekin.reserve(sizeof(double) * nsys) != hipSuccess || e->epot.reserve(sizeof(double) * nsys) != hipSuccess) {
         nbh_ens_destroy(e);
         return fail(NBH_ERR_ALLOC, "ensemble device allocation failed (%lld bodies)", total);
@@ -588,7 +632,7 @@ int nbh_ens_destroy(nbh_ensemble* e)
     (void)hipStreamSynchronize(e->ctx->stream);
     drop_graph(e);
     for (DevBuf* b : {&e->sys, &e->state, &e->f0, &e->f1, &e->items, &e->reduces, &e->partial, &e->counter,
-                      &e->dt, &e->ekin, &e->epot, &e->symitems})
+                      &e->dt, &e->ekin, &e->epot, &e->symitems, &e->vref})
         b->release();
     delete e;
     return NBH_OK;

@@ -25,6 +25,29 @@ struct Forces {
     double* c[6];
 };
 
+// Whether a force evaluation at softening eps2 must drop the self pair
+// explicitly (the MASK kernels).  With d = u = 0 the self pair adds
+// m r^3 * 0 = 0 as long as r^3 = eps2^-3/2 and m r^3 are finite; below these
+// thresholds they could overflow the pair arithmetic's type (FP32: eps2 <
+// ~2e-26 for m = 1, and (float)eps2 rounds to 0 below 1.4e-45; FP64: eps2 <
+// ~3e-206), and inf * 0 would put NaN into every target.  The thresholds
+// keep m r^3 finite for masses up to 3e8 (FP32) and 1e38 (FP64).
+inline bool self_mask(int precision, double eps2)
+{
+    return precision == 0 ? eps2 < 1e-180 : (float)eps2 < 1e-20f;
+}
+
+// Scratch of the reference-velocity reduction (doubles): partial sums of up
+// to vref_parts() workgroups, then the reference velocity itself at
+// vref_offset().  Mixed precision only.
+constexpr int kVrefParts = 256;
+constexpr size_t vref_work_doubles() { return 4 * (size_t)kVrefParts + 4; }
+constexpr size_t vref_offset() { return 4 * (size_t)kVrefParts; }
+// Centre-of-mass velocity of bodies 0..n-1 (sum m v / sum m; 0 when every
+// mass is 0) into work[vref_offset() .. +3), summed in a fixed order
+// (deterministic).  Graph-safe: kernels only.
+hipError_t launch_vref(const State& s, int n, double* work, hipStream_t st);
+
 // One force evaluation's launch shape.
 struct Shape {
     int block;   // lanes per workgroup (= the j-tile length of the LDS kernels): 64, 128 or 256
@@ -43,18 +66,24 @@ struct Shape {
 // rec: with shape.scalar, a device buffer of source_record_bytes(precision,
 // n) bytes for the packed source records (unused, may be null, otherwise).
 size_t source_record_bytes(int precision, int n);
-// Packs sources 0..n-1 of s into rec.
-hipError_t pack_source_records(int precision, const State& s, int n, void* rec, hipStream_t st);
+// Packs sources 0..n-1 of s into rec (mixed: velocities less vref, the 3
+// device doubles of the reference velocity).
+hipError_t pack_source_records(int precision, const State& s, int n, const double* vref, void* rec,
+                               hipStream_t st);
+// vwork: mixed precision, vref_work_doubles() doubles of device memory
+// for the reference velocity (computed first, from the n targets); unused
+// (may be null) in FP64.
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out,
-                        double* partial, void* rec, hipStream_t st, hipEvent_t ev0,
+                        double* partial, void* rec, double* vwork, hipStream_t st, hipEvent_t ev0,
                         hipEvent_t ev1, int nj = 0);
 
 // Directed pair kernel restricted to the diagonal blocks: target i gets the
 // sources of its own block [B*(i/B), B*(i/B)+B), self-pair excluded; the
 // outputs are overwritten (the last block may be partial).  B % 256 == 0.
+// vref: the reference velocity (3 doubles on the device; mixed only).
 hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
-                                  const Forces& out, hipStream_t st);
+                                  const Forces& out, const double* vref, hipStream_t st);
 
 // Symmetric pair kernels (nbh_sym.hip): block-diagonal pass, then every
 // off-diagonal block pair once with the reaction added to the source side.
@@ -73,7 +102,7 @@ int sym_list_max(int precision, int n);  // longest source list of a full block
 int sym_chunk(int precision, int n, int cus);
 long sym_workgroups(int precision, int n, int chunk);
 hipError_t launch_force_sym(int precision, const State& s, int n, double eps2,
-                            int chunk, const Forces& out, hipStream_t st,
+                            int chunk, const Forces& out, const double* vref, hipStream_t st,
                             hipEvent_t ev0, hipEvent_t ev1);
 
 // The ensemble's symmetric kernel cuts systems into blocks of its own size.
@@ -84,13 +113,14 @@ bool ens_sym_supported(int precision, int n);
 // block P of the system occupying [off, off + n) against entries i0 .. i1 of
 // its source list (blocks P+1 .. P+H mod nr, then the partial block if any),
 // with the system's eps2.  The outputs must already hold the block-diagonal
-// sums: the items add to them atomically.
+// sums: the items add to them atomically.  sys indexes the per-system
+// reference velocities vref[3 * sys ..] (mixed only).
 struct SymItem {
-    int off, n, P, i0, i1;
+    int off, n, P, i0, i1, sys;
     double eps2;
 };
 hipError_t launch_ens_sym(int precision, const State& s, const Forces& out,
-                          const SymItem* items, int nitems, hipStream_t st);
+                          const SymItem* items, int nitems, const double* vref, hipStream_t st);
 
 // Hermite predictor in place (x, v) from forces f0 with dt read from *dt.
 hipError_t launch_predict(int n, const double* dt, double* x, double* y,

@@ -32,6 +32,7 @@
 #include "nbh_internal.h"
 #include "nbh_pair.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace nbh {
@@ -48,6 +49,7 @@ struct PairArgs {
     Forces out;
     size_t split_stride;  // element offset between j-split partial planes
     int diag;             // > 0: sources are the target's own block of diag bodies
+    const double* vref;   // mixed: reference velocity (3 doubles), subtracted before rounding
 };
 
 // Source range of this workgroup: its j-split, or in the block-diagonal pass
@@ -143,6 +145,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_mixed_lds(PairArgs a)
     const int t = threadIdx.x;
     const int base = blockIdx.x * BLOCK * IPL + t;
     const float eps2 = (float)a.eps2;
+    const VRef vr = load_vref(a.vref);
     double xi[IPL], yi[IPL], zi[IPL];
     float vxi[IPL], vyi[IPL], vzi[IPL];
     float hi[IPL][6], lo[IPL][6];
@@ -151,7 +154,9 @@ __global__ __launch_bounds__(BLOCK) void k_force_mixed_lds(PairArgs a)
         const int i = base + q * BLOCK;
         const int ic = i < a.n ? i : a.n - 1;
         xi[q] = a.s.x[ic]; yi[q] = a.s.y[ic]; zi[q] = a.s.z[ic];
-        vxi[q] = (float)a.s.vx[ic]; vyi[q] = (float)a.s.vy[ic]; vzi[q] = (float)a.s.vz[ic];
+        vxi[q] = (float)(a.s.vx[ic] - vr.x);
+        vyi[q] = (float)(a.s.vy[ic] - vr.y);
+        vzi[q] = (float)(a.s.vz[ic] - vr.z);
 #pragma unroll
         for (int c = 0; c < 6; ++c) hi[q][c] = lo[q][c] = 0.f;
     }
@@ -159,14 +164,14 @@ __global__ __launch_bounds__(BLOCK) void k_force_mixed_lds(PairArgs a)
     int jbeg, jend;
     source_range<BLOCK, IPL>(a, jbeg, jend);
     JBMX st{};
-    if (jbeg + t < jend) st = load_jbmx(a.s, jbeg + t);
+    if (jbeg + t < jend) st = load_jbmx(a.s, jbeg + t, vr);
     tile[0][t] = st;
     __syncthreads();
     int buf = 0;
     for (int jt = jbeg; jt < jend; jt += BLOCK) {
         const int cnt = min(BLOCK, jend - jt);
         const bool more = jt + BLOCK < jend;
-        if (more && jt + BLOCK + t < jend) st = load_jbmx(a.s, jt + BLOCK + t);
+        if (more && jt + BLOCK + t < jend) st = load_jbmx(a.s, jt + BLOCK + t, vr);
         const JBMX* tb = tile[buf];
         for (int k0 = 0; k0 < cnt; k0 += kFold) {
             float acc[IPL][6];
@@ -220,10 +225,60 @@ __global__ __launch_bounds__(kOBlock) void k_pack_f64(State s, int n, JB64* __re
     if (j < n) rec[j] = load_jb64(s, j);
 }
 
-__global__ __launch_bounds__(kOBlock) void k_pack_mixed(State s, int n, JBMX* __restrict__ rec)
+__global__ __launch_bounds__(kOBlock) void k_pack_mixed(State s, int n, const double* __restrict__ vref,
+                                                        JBMX* __restrict__ rec)
 {
     const int j = blockIdx.x * kOBlock + threadIdx.x;
-    if (j < n) rec[j] = load_jbmx(s, j);
+    if (j < n) rec[j] = load_jbmx(s, j, load_vref(vref));
+}
+
+// Centre-of-mass velocity, first pass: workgroup g sums m, m vx, m vy, m vz
+// over its contiguous share of the bodies (tree order in LDS) into
+// work[4g .. 4g+3].
+__global__ __launch_bounds__(kOBlock) void k_vref_part(State s, int n, double* __restrict__ work)
+{
+    __shared__ double red[4][kOBlock];
+    const int t = threadIdx.x;
+    const int chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int j0 = blockIdx.x * chunk, j1 = min(n, j0 + chunk);
+    double m = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    for (int j = j0 + t; j < j1; j += kOBlock) {
+        const double mj = s.m[j];
+        m += mj;
+        px = fma(mj, s.vx[j], px);
+        py = fma(mj, s.vy[j], py);
+        pz = fma(mj, s.vz[j], pz);
+    }
+    red[0][t] = m; red[1][t] = px; red[2][t] = py; red[3][t] = pz;
+    __syncthreads();
+    for (int h = kOBlock / 2; h > 0; h >>= 1) {
+        if (t < h)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) red[c][t] += red[c][t + h];
+        __syncthreads();
+    }
+    if (t < 4) work[4 * blockIdx.x + t] = red[t][0];
+}
+
+// Second pass: the parts added in tree order; vref = sum m v / sum m, or 0
+// when the bodies carry no mass.
+__global__ __launch_bounds__(kVrefParts) void k_vref_final(double* __restrict__ work, int parts)
+{
+    __shared__ double red[4][kVrefParts];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) red[c][t] = t < parts ? work[4 * t + c] : 0.0;
+    __syncthreads();
+    for (int h = kVrefParts / 2; h > 0; h >>= 1) {
+        if (t < h)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) red[c][t] += red[c][t + h];
+        __syncthreads();
+    }
+    if (t < 3) {
+        const double m = red[0][0];
+        work[vref_offset() + t] = m > 0.0 ? red[t + 1][0] / m : 0.0;
+    }
 }
 
 // Scalar-load kernels.  BLOCK lanes per workgroup, IPL targets per lane
@@ -280,6 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a, const JBMX* _
     const int t = threadIdx.x;
     const int base = blockIdx.x * BLOCK * IPL + t;
     const float eps2 = (float)a.eps2;
+    const VRef vr = load_vref(a.vref);
     double xi[IPL], yi[IPL], zi[IPL];
     float vxi[IPL], vyi[IPL], vzi[IPL];
     float hi[IPL][6], lo[IPL][6];
@@ -288,7 +344,9 @@ __global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a, const JBMX* _
         const int i = base + q * BLOCK;
         const int ic = i < a.n ? i : a.n - 1;
         xi[q] = a.s.x[ic]; yi[q] = a.s.y[ic]; zi[q] = a.s.z[ic];
-        vxi[q] = (float)a.s.vx[ic]; vyi[q] = (float)a.s.vy[ic]; vzi[q] = (float)a.s.vz[ic];
+        vxi[q] = (float)(a.s.vx[ic] - vr.x);
+        vyi[q] = (float)(a.s.vy[ic] - vr.y);
+        vzi[q] = (float)(a.s.vz[ic] - vr.z);
 #pragma unroll
         for (int c = 0; c < 6; ++c) hi[q][c] = lo[q][c] = 0.f;
     }
@@ -658,20 +716,33 @@ size_t source_record_bytes(int precision, int n)
     return (precision == 0 ? sizeof(JB64) : sizeof(JBMX)) * (size_t)n;
 }
 
-hipError_t pack_source_records(int precision, const State& s, int n, void* rec, hipStream_t st)
+hipError_t pack_source_records(int precision, const State& s, int n, const double* vref, void* rec,
+                               hipStream_t st)
 {
-    if (!rec) return hipErrorInvalidValue;
+    if (!rec || (precision != 0 && !vref)) return hipErrorInvalidValue;
     if (precision == 0)
         k_pack_f64<<<(n + kOBlock - 1) / kOBlock, kOBlock, 0, st>>>(s, n, static_cast<JB64*>(rec));
     else
-        k_pack_mixed<<<(n + kOBlock - 1) / kOBlock, kOBlock, 0, st>>>(s, n, static_cast<JBMX*>(rec));
+        k_pack_mixed<<<(n + kOBlock - 1) / kOBlock, kOBlock, 0, st>>>(s, n, vref, static_cast<JBMX*>(rec));
+    return hipGetLastError();
+}
+
+hipError_t launch_vref(const State& s, int n, double* work, hipStream_t st)
+{
+    if (!work || n < 1) return hipErrorInvalidValue;
+    const int parts = std::min(kVrefParts, oblocks(n));
+    k_vref_part<<<parts, kOBlock, 0, st>>>(s, n, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_vref_final<<<1, kVrefParts, 0, st>>>(work, parts);
     return hipGetLastError();
 }
 
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out, double* partial,
-                        void* rec, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int nj)
+                        void* rec, double* vwork, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int nj)
 {
+    if (precision != 0 && !vwork) return hipErrorInvalidValue;
     PairArgs a;
     a.s = s;
     a.n = n;
@@ -679,9 +750,13 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
     a.jchunk = shape.jchunk;
     a.eps2 = eps2;
     a.diag = 0;
+    a.vref = precision != 0 ? vwork + vref_offset() : nullptr;
+    hipError_t e;
+    if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
+    if (precision != 0 && (e = launch_vref(s, n, vwork, st)) != hipSuccess) return e;
     if (shape.sym_chunk > 0) {
         if (nj > 0) return hipErrorInvalidValue;
-        return launch_force_sym(precision, s, n, eps2, shape.sym_chunk, out, st, ev0, ev1);
+        return launch_force_sym(precision, s, n, eps2, shape.sym_chunk, out, a.vref, st, nullptr, ev1);
     }
     if (shape.jsplit > 1) {
         // partial plane for (component c, split k) starts at (c*S + k)*n
@@ -691,11 +766,9 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
         a.out = out;
         a.split_stride = 0;
     }
-    const bool mask = (eps2 == 0.0);
+    const bool mask = self_mask(precision, eps2);
     const void* r = shape.scalar ? rec : nullptr;
-    hipError_t e;
-    if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
-    if (shape.scalar && (e = pack_source_records(precision, s, a.nj, rec, st)) != hipSuccess) return e;
+    if (shape.scalar && (e = pack_source_records(precision, s, a.nj, a.vref, rec, st)) != hipSuccess) return e;
     const int key = shape.block * 10 + shape.ipl;
     switch (key) {
     case 641: e = launch_pair<64, 1>(precision, mask, a, shape.jsplit, r, st); break;
@@ -716,10 +789,10 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
 }
 
 hipError_t launch_force_blockdiag(int precision, const State& s, int n, double eps2, int B,
-                                  const Forces& out, hipStream_t st)
+                                  const Forces& out, const double* vref, hipStream_t st)
 {
     constexpr int BLOCK = 256;
-    if (B <= 0 || B % BLOCK != 0) return hipErrorInvalidValue;
+    if (B <= 0 || B % BLOCK != 0 || (precision != 0 && !vref)) return hipErrorInvalidValue;
     PairArgs a;
     a.s = s;
     a.n = n;
@@ -729,7 +802,8 @@ hipError_t launch_force_blockdiag(int precision, const State& s, int n, double e
     a.out = out;
     a.split_stride = 0;
     a.diag = B;
-    return launch_pair<BLOCK, 1>(precision, eps2 == 0.0, a, 1, nullptr, st);  // LDS tiles: 512 sources each
+    a.vref = vref;
+    return launch_pair<BLOCK, 1>(precision, self_mask(precision, eps2), a, 1, nullptr, st);  // LDS tiles
 }
 
 hipError_t launch_predict(int n, const double* dt, double* x, double* y, double* z,

@@ -26,27 +26,18 @@ struct __align__(16) JBMX {
     double x, y, z, pad;
 };
 
-// s^-1/2 in FP64: v_rsq_f64 seed refined by one third-order step,
+// s^-1/2 in FP64 for every FP64 kernel: the v_rsq_f64 seed (measured
+// relative error <= 5.2e-8 on gfx950) refined by one third-order step,
 // y' = y + y e (1/2 + 3/8 e) with e = 1 - s y^2.  The seed's relative error
-// d becomes O(d^3) plus the rounding of e, i.e. about one ulp.
+// d becomes O(d^3) ~ 1e-22 plus the rounding of e and of the last fma, i.e.
+// about one ulp -- closer to 1/sqrt(s) than the reference's InvSqrt2D
+// (src/utils.cpp:155-165, up to 4.3e-16), which a plain Newton step
+// (d -> 1.5 d^2 ~ 4e-15) is not.
 __device__ __forceinline__ double rsqrt_f64(double s)
 {
     const double y = __builtin_amdgcn_rsq(s);
     const double e = fma(-s, y * y, 1.0);
     return fma(y * e, fma(e, 0.375, 0.5), y);
-}
-
-// s^-1/2 in FP64 for the pair kernels: v_rsq_f64 seed (measured relative
-// error <= 5.2e-8 on gfx950) refined by one Newton step y' = y + y e / 2,
-// e = 1 - s y^2.  Measured over 4M log-uniform s in [1e-12, 1e12]: max
-// relative error 4.1e-15, mean -1.5e-16 -- one instruction fewer than
-// rsqrt_f64, and well inside the per-body 1e-12 FP64 parity bar (the
-// reference's own InvSqrt2D errs by up to 4.3e-16).
-__device__ __forceinline__ double rsqrt_f64_pair(double s)
-{
-    const double y = __builtin_amdgcn_rsq(s);
-    const double e = fma(-s, y * y, 1.0);
-    return fma(y * e, 0.5, y);
 }
 
 // One source body gathered from the SoA state (coalesced when lanes take
@@ -65,12 +56,28 @@ __device__ __forceinline__ JB64 load_jb64(const State& s, int j)
     return b;
 }
 
-__device__ __forceinline__ JBMX load_jbmx(const State& s, int j)
+// Reference velocity of a mixed-precision evaluation: every velocity is
+// stored in FP32 as (float)(v - vref), with vref one FP64 velocity per system
+// (its centre-of-mass velocity, k_vref_*).  Velocity differences, the only
+// way velocities enter the jerk, are unchanged by the shift, but their FP32
+// rounding error then scales with the velocity dispersion about vref rather
+// than with the system's bulk velocity: a cluster moving at 1e3 times its
+// internal speeds keeps the jerk accuracy it has at rest.
+struct VRef {
+    double x, y, z;
+};
+
+__device__ __forceinline__ VRef load_vref(const double* __restrict__ p)
+{
+    return VRef{p[0], p[1], p[2]};
+}
+
+__device__ __forceinline__ JBMX load_jbmx(const State& s, int j, const VRef& vr)
 {
     JBMX b;
-    b.vx = (float)s.vx[j];
-    b.vy = (float)s.vy[j];
-    b.vz = (float)s.vz[j];
+    b.vx = (float)(s.vx[j] - vr.x);
+    b.vy = (float)(s.vy[j] - vr.y);
+    b.vz = (float)(s.vz[j] - vr.z);
     b.m = (float)s.m[j];
     b.x = s.x[j];
     b.y = s.y[j];
@@ -95,12 +102,12 @@ __device__ __forceinline__ void pair_f64(const JB64& b, int j, int i,
     double s = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
     const double vr = fma(dx, ux, fma(dy, uy, dz * uz));
     double mj = b.m;
-    if (MASK) {  // eps2 == 0: drop the self pair instead of dividing by 0
+    if (MASK) {  // eps2 too small for a finite self pair (self_mask): drop it
         const bool self = (j == i);
         s = self ? 1.0 : s;
         mj = self ? 0.0 : mj;
     }
-    const double r = rsqrt_f64_pair(s);
+    const double r = rsqrt_f64(s);
     const double r2 = r * r;
     const double mr3 = mj * r * r2;
     const double al = -3.0 * vr * r2;

@@ -110,21 +110,23 @@ struct SymMixed {
     struct Tot {
         float hi[K][6], lo[K][6];
     };
-    static __device__ __forceinline__ Rec load_rec(const State& s, int j, int n)
+    static __device__ __forceinline__ Rec load_rec(const State& s, int j, int n, const VRef& vr)
     {
-        if (j < n) return load_jbmx(s, j);
+        if (j < n) return load_jbmx(s, j, vr);
         Rec b{};
         b.x = pad_x(j);
         return b;
     }
-    static __device__ __forceinline__ void load_targets(const State& s, int ibase, Targets& I)
+    static __device__ __forceinline__ void load_targets(const State& s, int ibase, Targets& I, const VRef& vr)
     {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = ibase + k * 64;
             I.x[k] = s.x[i]; I.y[k] = s.y[i]; I.z[k] = s.z[i];
             I.m[k] = (float)s.m[i];
-            I.vx[k] = (float)s.vx[i]; I.vy[k] = (float)s.vy[i]; I.vz[k] = (float)s.vz[i];
+            I.vx[k] = (float)(s.vx[i] - vr.x);
+            I.vy[k] = (float)(s.vy[i] - vr.y);
+            I.vz[k] = (float)(s.vz[i] - vr.z);
         }
     }
     template <int R>
@@ -214,7 +216,7 @@ struct SymF64 {
     struct Tot {
         double t[K][6];
     };
-    static __device__ __forceinline__ Rec load_rec(const State& s, int j, int n)
+    static __device__ __forceinline__ Rec load_rec(const State& s, int j, int n, const VRef&)
     {
         Rec b{};
         if (j < n) {
@@ -225,7 +227,7 @@ struct SymF64 {
         }
         return b;
     }
-    static __device__ __forceinline__ void load_targets(const State& s, int ibase, Targets& I)
+    static __device__ __forceinline__ void load_targets(const State& s, int ibase, Targets& I, const VRef&)
     {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -251,7 +253,7 @@ struct SymF64 {
             const double uz = b.vz - I.vz[k];
             const double s = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
             const double vr = fma(dx, ux, fma(dy, uy, dz * uz));
-            const double r = rsqrt_f64_pair(s);
+            const double r = rsqrt_f64(s);
             const double r2 = r * r;
             const double r3 = r * r2;
             const double mj3 = b.m * r3;
@@ -308,6 +310,7 @@ struct SymArgs {
     int chunk;    // source-list entries per workgroup
     double eps2;
     Forces out;   // initialised by the block-diagonal pass
+    const double* vref;  // mixed: reference velocity (3 doubles)
 };
 
 // Full block P of one system (bodies 0..n-1 of s and out) against entries
@@ -315,7 +318,7 @@ struct SymArgs {
 // target sums go to the outputs every FL tiles; FL = 0: into per-lane totals.
 template <template <int> class TT, int K, int W, int FL>
 __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& out, int n, double eps2d, int P,
-                                                int i0, int i1)
+                                                int i0, int i1, const VRef& vr)
 {
     using T = TT<K>;
     using Sum = typename T::Sum;
@@ -336,7 +339,7 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
     const int ibase = P * B + w * 64 * K + lane;
 
     typename T::Targets I;
-    T::load_targets(s, ibase, I);
+    T::load_targets(s, ibase, I, vr);
     // first source of tile tt of this workgroup's range
     auto tile_src = [&](int tt) {
         const int i = i0 + tt / TPB;
@@ -348,7 +351,7 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
 
     typename T::Rec st{};
     if (t < kSymTile) {
-        st = T::load_rec(s, tile_src(0) + t, n);
+        st = T::load_rec(s, tile_src(0) + t, n, vr);
         dup[0][t >> 4][t & 15] = st;
         dup[0][t >> 4][(t & 15) + 16] = st;
     }
@@ -397,7 +400,7 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
         // the next tile is loaded after the arithmetic, not held in registers
         // across it (12 VGPRs); the other resident workgroups cover the load
         if (more && t < kSymTile) {
-            st = T::load_rec(s, tile_src(tt + 1) + t, n);
+            st = T::load_rec(s, tile_src(tt + 1) + t, n, vr);
             dup[buf ^ 1][t >> 4][t & 15] = st;
             dup[buf ^ 1][t >> 4][(t & 15) + 16] = st;
         }
@@ -449,23 +452,25 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_wave
     const int i0 = blockIdx.y * a.chunk;
     if (i0 >= len) return;  // uniform over the workgroup, before any barrier
     const int i1 = min(len, i0 + a.chunk) - 1;
-    sym_block_pairs<TT, K, W, TT<K>::kFlushTiles>(a.s, a.out, a.n, a.eps2, P, i0, i1);
+    const VRef vr = a.vref ? load_vref(a.vref) : VRef{0.0, 0.0, 0.0};
+    sym_block_pairs<TT, K, W, TT<K>::kFlushTiles>(a.s, a.out, a.n, a.eps2, P, i0, i1, vr);
 }
 
 // Many systems: one workgroup per item of a list (system offset and size,
 // block P and a range of its source list), systems' arrays concatenated.
 template <template <int> class TT, int K, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sym_waves<TT<K>, W>(), 8))) void k_ens_sym(
-    State s, Forces out, const SymItem* __restrict__ items)
+    State s, Forces out, const SymItem* __restrict__ items, const double* __restrict__ vref)
 {
     const SymItem it = items[blockIdx.x];
+    const VRef vr = vref ? load_vref(vref + 3 * it.sys) : VRef{0.0, 0.0, 0.0};
     State q = s;
     q.x += it.off; q.y += it.off; q.z += it.off; q.m += it.off;
     q.vx += it.off; q.vy += it.off; q.vz += it.off;
     Forces o = out;
 #pragma unroll
     for (int c = 0; c < 6; ++c) o.c[c] += it.off;
-    sym_block_pairs<TT, K, W, 0>(q, o, it.n, it.eps2, it.P, it.i0, it.i1);
+    sym_block_pairs<TT, K, W, 0>(q, o, it.n, it.eps2, it.P, it.i0, it.i1, vr);
 }
 
 // Launch shape per precision: targets per lane and waves per workgroup of
@@ -514,30 +519,32 @@ long sym_workgroups(int precision, int n, int chunk)
 }
 
 hipError_t launch_ens_sym(int precision, const State& s, const Forces& out, const SymItem* items, int nitems,
-                          hipStream_t st)
+                          const double* vref, hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
+    if (precision != 0 && !vref) return hipErrorInvalidValue;
     if (precision == 0)
-        k_ens_sym<SymF64, kF64EnsK, kF64EnsW><<<nitems, 64 * kF64EnsW, 0, st>>>(s, out, items);
+        k_ens_sym<SymF64, kF64EnsK, kF64EnsW><<<nitems, 64 * kF64EnsW, 0, st>>>(s, out, items, nullptr);
     else
-        k_ens_sym<SymMixed, kMixEnsK, kMixEnsW><<<nitems, 64 * kMixEnsW, 0, st>>>(s, out, items);
+        k_ens_sym<SymMixed, kMixEnsK, kMixEnsW><<<nitems, 64 * kMixEnsW, 0, st>>>(s, out, items, vref);
     return hipGetLastError();
 }
 
 hipError_t launch_force_sym(int precision, const State& s, int n, double eps2, int chunk, const Forces& out,
-                            hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+                            const double* vref, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (!sym_supported(precision, n) || chunk < 1) return hipErrorInvalidValue;
+    if (!sym_supported(precision, n) || chunk < 1 || (precision != 0 && !vref)) return hipErrorInvalidValue;
     const int B = sym_block(precision);
     hipError_t e;
     if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
-    if ((e = launch_force_blockdiag(precision, s, n, eps2, B, out, st)) != hipSuccess) return e;
+    if ((e = launch_force_blockdiag(precision, s, n, eps2, B, out, vref, st)) != hipSuccess) return e;
     SymArgs a;
     a.s = s;
     a.n = n;
     a.chunk = chunk;
     a.eps2 = eps2;
     a.out = out;
+    a.vref = vref;
     const int per = sym_list_max(precision, n);
     const dim3 grid(n / B, (per + chunk - 1) / chunk);
     if (precision == 0)

@@ -378,3 +378,42 @@ def test_force_device_pointers_symmetric(oracle, nb, precision):
         check64(out, ref)
     else:
         assert np.median(body_relerr(out[:3], ref[:3])) <= 1e-6
+
+
+# eps2 > 0 but too small for the self pair's m eps2^-3/2 to stay finite in
+# the pair arithmetic's type (FP32 below ~2e-26; FP64 below ~3e-206; 1e-40
+# is an FP32 denormal, 1e-50 rounds to 0 in FP32): the self pair must be
+# dropped explicitly there too, or inf * 0 turns every force into NaN.
+@pytest.mark.parametrize("precision,eps2", [("mixed", 1e-30), ("mixed", 1e-40), ("mixed", 1e-50),
+                                            ("fp64", 1e-200), ("fp64", 1e-300)])
+@pytest.mark.parametrize("pairing,sources", [("directed", "lds"), ("directed", "scalar"), ("symmetric", "auto")])
+def test_tiny_softening_drops_self_pair(nb, oracle, precision, eps2, pairing, sources):
+    n = 2 * 768 + 300  # two symmetric blocks and a partial one
+    _banner(f"tiny eps2 {precision} {eps2:g} {pairing} {sources}")
+    s = nb.plummer(n, 91)
+    with nb.Backend(precision, eps2, pairing=pairing, sources=sources) as b:
+        assert b.launch_shape(n)["pairing"] == pairing
+        f = b.force(s)  # raises ERR_NONFINITE if a NaN got through
+    ref = oracle.force(s, eps2)
+    if precision == "fp64":
+        check64(f, ref)
+    else:
+        check_mixed(f, ref)
+
+
+@pytest.mark.parametrize("precision,eps2", [("mixed", 1e-30), ("fp64", 1e-200)])
+@pytest.mark.parametrize("pairing", ["directed", "symmetric"])
+def test_tiny_softening_ensemble(nb, oracle, precision, eps2, pairing):
+    counts = np.array([40, 600, 1100], dtype=np.int32)
+    eps = np.array([1e-3, eps2, eps2])
+    state = nb.ensemble_state(counts, 92)
+    with nb.Backend(precision, pairing=pairing) as b, nb.Ensemble(b, counts, eps) as e:
+        e.upload(state)
+        _, f = e.download()
+        assert np.isfinite(f).all()
+        for k in range(len(counts)):
+            fk, ref = e.system(f, k), oracle.force(e.system(state, k), eps[k])
+            if precision == "fp64":
+                check64(fk, ref)
+            else:
+                check_mixed(fk, ref)

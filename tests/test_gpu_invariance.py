@@ -129,3 +129,76 @@ def test_rotation_equivariance(nb, precision, pairing, sources):
     else:
         assert np.median(ea) <= 1e-6 and np.median(ej) <= 1e-5, \
             f"acc median {np.median(ea):.2e} jerk median {np.median(ej):.2e}"
+
+
+# A bulk velocity ~2e3 times the spheres' internal speeds (velocity
+# dispersion ~0.4 in Henon units).
+BOOST = np.array([1.0e3, -4.0e2, 7.0e2])
+
+
+def jerk_of(s, idx, eps2, u_of):
+    """CPU jerk of targets idx of s with FP64-formed position differences
+    rounded to FP32 (the mixed kernels' d) and velocity differences from
+    u_of(idx) -> (3, targets, sources); everything else in FP64."""
+    d = (s[0:3][:, None, :] - s[0:3][:, idx, None]).astype(np.float32).astype(np.float64)
+    u = u_of(idx)
+    r2 = (d ** 2).sum(0) + eps2
+    r = 1.0 / np.sqrt(r2)
+    mr3 = s[3][None, :] * r ** 3
+    mr3[np.arange(idx.size), idx] = 0.0
+    al = -3.0 * (d * u).sum(0) * r * r
+    return (mr3[None] * (u + al[None] * d)).sum(2)
+
+
+@pytest.fixture(scope="module")
+def boosted(nb, oracle):
+    s = nb.plummer(N, 34)
+    t = s.copy()
+    t[4:7] += BOOST[:, None]
+    return s, t, oracle.force(t, EPS2, nthreads=8)
+
+
+def test_fp32_velocities_would_fail_under_a_boost(boosted):
+    """The mistake the bulk-velocity test catches, on the CPU: velocities
+    rounded to FP32 before the subtraction (as the reference's _F32 build
+    does not: it forms v_j - v_i in FP64, src/kernel.cpp:1131-1133) lose the
+    jerk once the system moves, while velocities taken relative to the
+    centre-of-mass velocity in FP64 and then rounded keep it."""
+    s, t, ref = boosted
+    idx = np.arange(0, N, 37)
+    v32 = t[4:7].astype(np.float32)
+    vc = (t[3] * t[4:7]).sum(1) / t[3].sum()
+    w32 = (t[4:7] - vc[:, None]).astype(np.float32)
+    wrong = jerk_of(t, idx, EPS2, lambda i: (v32[:, None, :] - v32[:, i, None]).astype(np.float64))
+    right = jerk_of(t, idx, EPS2, lambda i: (w32[:, None, :] - w32[:, i, None]).astype(np.float64))
+    e_wrong = np.median(body_relerr(wrong, ref[3:, idx]))
+    e_right = np.median(body_relerr(right, ref[3:, idx]))
+    assert e_wrong > 1e-5, f"FP32-rounded velocities only err {e_wrong:.2e}"
+    assert e_right <= 1e-6, f"centre-of-mass-relative velocities err {e_right:.2e}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("pairing,sources", KERNELS)
+def test_bulk_velocity(nb, boosted, precision, pairing, sources):
+    """Galilean invariance: the same sphere at rest and moving at BOOST.
+    Forces depend on velocity differences only, so both must meet the bar
+    against the oracle of the moving sphere; in mixed precision the jerk must
+    also keep its at-rest accuracy (median <= 1e-6, against ~2e-5 with
+    FP32-rounded velocities, test_fp32_velocities_would_fail_under_a_boost)
+    and agree with the at-rest evaluation."""
+    s, t, ref = boosted
+    _banner(f"bulk velocity {precision} {pairing} {sources}")
+    f = force(nb, precision, pairing, sources, t)
+    check(precision, f, ref)
+    g = force(nb, precision, pairing, sources, s)
+    ej = body_relerr(f[3:], ref[3:])
+    dj = body_relerr(f[3:], g[3:])
+    da = body_relerr(f[:3], g[:3])
+    _banner(f"  jerk vs oracle median {np.median(ej):.2e}; moving vs at rest: acc max {da.max():.2e}, "
+            f"jerk median {np.median(dj):.2e} max {dj.max():.2e}")
+    if precision == "fp64":
+        assert da.max() <= 1e-12 and dj.max() <= 1e-9
+    else:
+        assert np.median(ej) <= 1e-6, f"jerk median {np.median(ej):.2e}"
+        assert da.max() <= 1e-6 and np.median(dj) <= 1e-7, (da.max(), np.median(dj))
