@@ -132,6 +132,30 @@ int nbh_force_device(nbh_ctx *ctx, int n,
                      double *ax, double *ay, double *az,
                      double *jx, double *jy, double *jz, void *stream);
 
+/* nbh_force_device without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory during the call (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, semantics and kernels as nbh_force_device,
+ * except that the evaluation's working memory (reference-velocity reduction,
+ * packed source records, j-split partial sums) lives in a static device
+ * array of 64 MiB that each GPU the library runs on holds from module load
+ * to process end, instead of in context buffers; the call makes no
+ * hipMalloc/hipFree or other allocation.  When the context's launch shape
+ * for n needs more than 64 MiB, the call narrows it (fewer j-splits, then
+ * LDS-tile sources, which need none of it); the result then equals
+ * nbh_force_device's to FP64 rounding of the split sums rather than bit for
+ * bit.  A j-split or source path forced by the config is kept, and a call
+ * whose forced shape does not fit fails with NBH_ERR_ARG.  Every call on one
+ * GPU shares that GPU's array: calls that may overlap on one GPU (other
+ * streams, other contexts, other threads) race and must be serialised by
+ * the caller.  Profiling events are not recorded. */
+int nbh_force_device_noalloc(nbh_ctx *ctx, int n,
+                             const double *x, const double *y, const double *z,
+                             const double *m, const double *vx, const double *vy,
+                             const double *vz,
+                             double *ax, double *ay, double *az,
+                             double *jx, double *jy, double *jz, void *stream);
+
 /* ---- device-resident integrator (NBody, include/nbody.h:24-59) ------ */
 
 /* Copies the state to HBM and evaluates the initial force

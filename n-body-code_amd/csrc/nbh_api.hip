@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -22,6 +23,40 @@ namespace nbh {
 namespace {
 thread_local std::string g_err;
 }
+
+// Scratch of nbh_force_device_noalloc: the reference-velocity reduction,
+// packed source records and j-split partial sums of one evaluation, in a
+// file-scope device array instead of context buffers, so that the call
+// allocates nothing.  Every GPU the library runs on holds its own copy,
+// from module load to process end.
+constexpr size_t kScratchBytes = (size_t)64 << 20;  // 64 MiB
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per device
+
+// Address of this device's copy of `scratch` (the current device must be
+// `device`), looked up once per device.  The array's name is a host-side
+// handle only; kernels and copies must be given this address.
+int scratch_setup(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return fail(NBH_ERR_ARG, "device %d beyond the scratch table (%d devices)", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(NBH_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return NBH_OK;
+}
+}  // namespace
 
 int fail(int code, const char* fmt, ...)
 {
@@ -219,13 +254,54 @@ int enqueue_force(nbh_ctx* c, const nbh::State& s, int n, const nbh::Forces& out
 
 bool bad_n(int n) { return n < 1; }
 
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Static-scratch bytes a force evaluation of shape sh needs: the
+// reference-velocity reduction, then (directed kernels) the packed source
+// records and the j-split partials, each 256-byte aligned.
+size_t scratch_need(int precision, int n, const nbh::Shape& sh)
+{
+    size_t need = align256(sizeof(double) * nbh::vref_work_doubles());
+    if (sh.sym_chunk > 0) return need;
+    if (sh.scalar) need += align256(nbh::source_record_bytes(precision, n));
+    if (sh.jsplit > 1) need += sizeof(double) * 6 * (size_t)sh.jsplit * n;
+    return need;
+}
+
+// Narrows sh until it fits the static scratch: fewer j-splits, then LDS
+// sources (which need no records; with one split the evaluation needs only
+// the reduction's few KiB).  A j-split or source path that the config
+// forces is not changed: if it does not fit, the call is rejected.
+int fit_scratch(const nbh_ctx* c, int n, nbh::Shape& sh)
+{
+    const int prec = c->cfg.precision;
+    if (scratch_need(prec, n, sh) <= nbh::kScratchBytes) return NBH_OK;
+    if (c->cfg.jsplit > 0 || c->cfg.sources != NBH_SRC_AUTO)
+        return fail(NBH_ERR_ARG,
+                    "n=%d with the configured j-split %d and %s sources needs %zu bytes of scratch; "
+                    "nbh_force_device_noalloc has %zu (leave jsplit and sources automatic, or use "
+                    "nbh_force_device)",
+                    n, sh.jsplit, sh.scalar ? "scalar-load" : "LDS-tile", scratch_need(prec, n, sh), nbh::kScratchBytes);
+    for (int scalar = sh.scalar; scalar >= 0; --scalar) {
+        for (int split = sh.jsplit; split >= 1; split = split > 1 ? split / 2 : 0) {
+            const nbh::Shape t = make_shape(n, sh.block, sh.ipl, split, scalar != 0);
+            if (scratch_need(prec, n, t) <= nbh::kScratchBytes) {
+                sh = t;
+                return NBH_OK;
+            }
+        }
+    }
+    return fail(NBH_ERR_ARG, "n=%d does not fit the %zu-byte scratch of nbh_force_device_noalloc", n,
+                nbh::kScratchBytes);
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* nbh_last_error(void) { return nbh::last_error(); }
 
-int nbh_version(void) { return 103; }
+int nbh_version(void) { return 104; }
 
 int nbh_device_info(int device, int* compute_units, int* wave_size, char* name, int name_len)
 {
@@ -271,9 +347,13 @@ int nbh_create(const nbh_config* cfg, nbh_ctx** out)
     if (e == hipSuccess) e = c->flag.reserve(sizeof(int) * 2);
     if (e == hipSuccess) e = c->vwork.reserve(sizeof(double) * nbh::vref_work_doubles());
     if (e != hipSuccess) {
-        (void)hipStreamDestroy(c->stream);
-        delete c;
+        nbh_destroy(c);
         return fail(NBH_ERR_ALLOC, "device allocation failed: %s", hipGetErrorString(e));
+    }
+    rc = nbh::scratch_setup(cfg->device, &c->scratch);
+    if (rc) {
+        nbh_destroy(c);
+        return rc;
     }
     *out = c;
     return NBH_OK;
@@ -557,6 +637,44 @@ int nbh_force_device(nbh_ctx* c, int n, const double* x, const double* y, const 
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
     // events are only timed on the context's own stream, which nbh_timing drains
     return enqueue_force(c, s, n, f, st, st == c->stream);
+}
+
+int nbh_force_device_noalloc(nbh_ctx* c, int n, const double* x, const double* y, const double* z,
+                             const double* m, const double* vx, const double* vy, const double* vz,
+                             double* ax, double* ay, double* az, double* jx, double* jy, double* jz,
+                             void* stream)
+{
+    if (!c) return fail(NBH_ERR_ARG, "context is NULL");
+    if (bad_n(n)) return fail(NBH_ERR_ARG, "n must be >= 1, got %d", n);
+    const double* ins[7] = {x, y, z, m, vx, vy, vz};
+    double* outs[6] = {ax, ay, az, jx, jy, jz};
+    for (auto p : ins)
+        if (!p) return fail(NBH_ERR_ARG, "input array is NULL");
+    for (auto p : outs)
+        if (!p) return fail(NBH_ERR_ARG, "output array is NULL");
+    if (!c->scratch) return fail(NBH_ERR_STATE, "context has no scratch address");
+    NBH_HIP(hipSetDevice(c->cfg.device));
+    nbh::Shape sh = pick_shape(c, n);
+    if (int rc = fit_scratch(c, n, sh)) return rc;
+    // layout: reduction | records | partials (every piece is fully written by
+    // the evaluation before it is read, so nothing needs clearing)
+    char* base = static_cast<char*>(c->scratch);
+    size_t off = align256(sizeof(double) * nbh::vref_work_doubles());
+    double* vwork = reinterpret_cast<double*>(base);
+    void* rec = nullptr;
+    if (sh.sym_chunk == 0 && sh.scalar) {
+        rec = base + off;
+        off += align256(nbh::source_record_bytes(c->cfg.precision, n));
+    }
+    double* part = (sh.sym_chunk == 0 && sh.jsplit > 1) ? reinterpret_cast<double*>(base + off) : nullptr;
+    nbh::State s{x, y, z, m, vx, vy, vz};
+    nbh::Forces f;
+    for (int k = 0; k < 6; ++k) f.c[k] = outs[k];
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const hipError_t e = nbh::launch_force(c->cfg.precision, s, n, c->cfg.eps2, sh, f, part, rec, vwork, st,
+                                           nullptr, nullptr);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "force launch failed: %s", hipGetErrorString(e));
+    return NBH_OK;
 }
 
 int nbh_upload(nbh_ctx* c, int n, const double* x, const double* y, const double* z,

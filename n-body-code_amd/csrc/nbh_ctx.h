@@ -63,6 +63,7 @@ struct nbh_ctx {
     nbh::DevBuf partial;  // 6 * jsplit * n
     nbh::DevBuf rec;      // packed source records: 64 B (FP64) / 48 B (mixed) per body
     nbh::DevBuf vwork;    // mixed: reference-velocity reduction, vref_work_doubles()
+    void* scratch = nullptr;  // this device's copy of the static scratch (nbh_force_device_noalloc)
     nbh::DevBuf flag;     // int
     nbh::DevBuf scal;     // scalars: [0] dt, [1] dt_next bits, [2..3] energy, [4..] spare
 

@@ -66,6 +66,7 @@ def lib():
     L.nbh_configure.argtypes = [vp, C.POINTER(Config)]
     L.nbh_force.argtypes = [vp, C.c_int] + [_D] * 13 + [_D]
     L.nbh_force_device.argtypes = [vp, C.c_int] + [vp] * 13 + [vp]
+    L.nbh_force_device_noalloc.argtypes = [vp, C.c_int] + [vp] * 13 + [vp]
     L.nbh_upload.argtypes = [vp, C.c_int] + [_D] * 7
     L.nbh_initial_dt.argtypes = [vp, C.c_double, _D]
     L.nbh_set_dt.argtypes = [vp, C.c_double, C.c_int, C.c_double]

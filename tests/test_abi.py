@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(nb):
 
 
 def test_version(nb):
-    assert nb.lib().nbh_version() == 103
+    assert nb.lib().nbh_version() == 104
 
 
 def test_reference_ic_matches_reference_fixture(nb):
