@@ -120,3 +120,26 @@ def test_save_then_load_continues_the_run(nb, tmp_path):
     # carries the forces of the predicted state: an O(dt^4) difference in the
     # forces, amplified by close pairs (5.9e-9 observed)
     assert np.abs(nb.state_load(b) - nb.state_load(c)).max() < 1e-7
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+def test_dropin_run_matches_oracle(nb, oracle, tmp_path, precision, tol):
+    """The drop-in path end to end: nbody_app's host NBody integrator calling
+    force_calculation (the HIP shim over nbh_force with host arrays), two
+    fixed-dt steps of the 1024-body Plummer sphere (seed 1, eps = 1/64,
+    dt = 1/512), saved and compared with the oracle's run of the reference
+    step sequence (src/nbody.cpp:224-289) from the same initial state:
+    positions and velocities to 1e-13 relative to their scale (FP64); the
+    mixed force's error carried through two steps (mixed)."""
+    out = tmp_path / "s.nbh"
+    r = run("--N", "1024", "--ic", "plummer", "--seed", "1", "--eps", "0.015625", "--dt", "0.001953125",
+            "--Ncycle", "2", "--energy", "physical", "--precision", precision, "--save", str(out))
+    assert r.returncode == 0, r.stderr
+    got = nb.state_load(str(out))
+    ref = oracle.run(nb.plummer(1024, 1), 0.015625 ** 2, 0.001953125, 2)["state"]
+    assert np.array_equal(got[3], ref[3])
+    dx = np.abs(got[0:3] - ref[0:3]).max() / np.abs(ref[0:3]).max()
+    dv = np.abs(got[4:7] - ref[4:7]).max() / np.abs(ref[4:7]).max()
+    print(f"[gpu-cli] {precision} drop-in run vs oracle: positions {dx:.2e}, velocities {dv:.2e}", flush=True)
+    assert dx <= tol and dv <= tol
