@@ -10,36 +10,52 @@ correct), run by the device-resident integrator.  Four legs run:
   fp64_symmetric -- FP64, each unordered pair evaluated once and applied to
                     both bodies (symmetric pairing, the reference's j < i
                     scheme), totals combined with FP64 atomics;
-  mixed          -- FP64 position differences rounded to FP32, FP32 r^2 /
-                    rsqrt / jerk, two-sum compensated FP32 sums, symmetric
-                    pairing: the headline `value`;
-  mixed_directed -- the mixed arithmetic on directed pairs.
+  mixed          -- FP64 position differences rounded to FP32 (velocities
+                    taken relative to the centre-of-mass velocity in FP64,
+                    then rounded), FP32 r^2 / rsqrt / jerk, symmetric
+                    pairing: FP32 sums over runs of 128 sources (targets) or
+                    one tile of 64 sources x 4 waves (sources), carried on in
+                    FP64 with FP64 atomic adds: the headline `value`;
+  mixed_directed -- the mixed arithmetic on directed pairs, FP32 sums over
+                    runs of 64 sources carried in an FP32 two-sum (hi, lo).
 `value` is the mixed leg's N^2 * steps / time.  It is valid only where its
 median per-body relative acceleration error against FP64 is <= 1e-6; that
 error is measured in the same run on the same state.  mixed_over_fp64 gives
 the like-for-like ratios (symmetric/symmetric, directed/directed) and the
 symmetric mixed leg over the directed FP64 drop-in.
 
-Roofline: the pair kernels are VALU-bound (no MFMA; ~1e4-1e5 flop per HBM
-byte).  Algorithmic work, for either pairing: 60 flop per directed pair (acc +
-jerk convention) over the N^2 directed pairs of an evaluation, the count used
-for the reference's own triangle loop as well.  achieved = that work / the
-kernels' mean launch time from HIP events recorded on their own stream in the
-timed region; peak = MI355X vector peak (FP32 157.3 TFLOP/s for mixed, FP64
-78.6 TFLOP/s for FP64).  For the symmetric kernels, which evaluate each
-unordered pair once, `executed` restates the rate with the work they do: 73
-flop per unordered pair (the 60 plus m_i r^3 and six multiply-adds for the
-reaction) and 60 per directed pair of the block-diagonal pass.
+Roofline: the pair kernels are VALU-bound (no MFMA; ~1e2-1e4 flop per HBM
+byte).  achieved = the flop the evaluation's kernels perform / their mean
+duration from HIP events recorded on their own stream around every force
+evaluation of the timed region; peak = MI355X vector peak (FP32 157.3
+TFLOP/s for mixed, FP64 78.6 TFLOP/s for FP64).  Performed work: directed
+kernels 60 flop per directed pair (acc + jerk convention) x N^2; symmetric
+kernels, which evaluate each unordered pair once, 73 flop per unordered pair
+(the 60 plus m_i r^3 and six multiply-adds for the reaction) plus 60 per
+directed pair of the block-diagonal pass -- so frac <= 1 is a utilisation.
+`effective_tflops` restates every leg by the convention that credits all
+N^2 directed pairs with 60 flop (the count the reference's own triangle
+loop is quoted on), which exceeds the vector peak for the symmetric FP64
+kernel; it is a rate, not a utilisation.  `traffic` is not measured by this
+run: it is the HBM bytes per launch that rocprofv3 FETCH_SIZE / WRITE_SIZE
+passes over this same program measured on the lease named in
+`traffic_source` (profiles/pmc_force_n1048576.md).
 
-cpu_baseline: the oracle's restatement of the reference FP64 force loop
-(src/kernel.cpp:1164-1224: Newton-3rd-law triangle, cyclic rows per thread,
-per-thread sums merged at the end) on the first M bodies of the same
-sphere, OpenMP threads on this host's cores, M sized for ~10-20 s.
+cpu_baseline: the reference's own CPU force_calculation, FP64
+(src/kernel.cpp:1164-1224) and _F32 (src/kernel.cpp:1101-1163), compiled
+from its sources into oracle/_ref, run as the reference runs it: under
+mpiexec, one MPI rank per core with one OpenMP thread, cyclic rows of the
+Newton's-third-law triangle per rank; the maximum over ranks of the
+force_calculation time, median of 3 repetitions, on the first M bodies of
+the same sphere (M sized for ~4 s per repetition).  Cores = the CPU share
+this process may use (OMP_NUM_THREADS, else the host's CPUs).
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -79,18 +95,24 @@ def calibration_copy(nbytes=256 << 20):
         hip.hipFree(b)
 
 
+TRAFFIC_FILE = os.path.join("profiles", "pmc_force_n1048576.md")
+
+
 def measured_traffic():
     """HBM bytes per launch of the pair kernels, as measured by rocprofv3
-    --pmc FETCH_SIZE / WRITE_SIZE passes (scaled by the calibration copy) and
-    recorded in profiles/pmc_force_n1048576.md; None when not recorded."""
-    path = os.path.join(ROOT, "profiles", "pmc_force_n1048576.md")
-    out = {}
+    --pmc FETCH_SIZE / WRITE_SIZE passes over this program (scaled by the
+    calibration copy) and recorded in profiles/pmc_force_n1048576.md, with
+    the lease line that file names; ({}, None) when not recorded."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    out, lease = {}, None
     if os.path.exists(path):
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch "):
                 _, kern, val = line.split()
                 out[kern] = float(val)
-    return out
+            elif line.startswith("measured_on "):
+                lease = line.split(" ", 1)[1].strip()
+    return out, lease
 
 
 def kname(prec, res):
@@ -104,8 +126,9 @@ def kname(prec, res):
     return f"k_force_{'f64' if prec == 'fp64' else 'mixed'}{lds}<{sh['block']},{sh['ipl']}>"
 
 
-def kernel_flop(shape):
-    """Algorithmic flop of one force evaluation: 60 per directed pair, N^2 pairs."""
+def convention_flop(shape):
+    """The 60-flop-per-directed-pair convention over all N^2 directed pairs of
+    an evaluation, whatever the pairing."""
     return FLOP_PER_PAIR * N * N
 
 
@@ -115,7 +138,9 @@ def sym_block(shape):
 
 
 def executed_flop(shape):
-    """Flop the launch shape's kernels perform for one evaluation."""
+    """Flop the launch shape's kernels perform for one evaluation: 60 per
+    directed pair for directed pairing; for symmetric pairing 73 per unordered
+    pair between blocks plus 60 per directed pair inside the blocks."""
     if shape.get("pairing") == "symmetric":
         b = sym_block(shape)
         inside = (N // b) * b * b + (N % b) ** 2      # ordered pairs inside blocks, self pairs included
@@ -152,7 +177,7 @@ def run_precision(nb, prec, state, steps, warmup, log, pairing="directed"):
         dropin_wall = time.perf_counter() - t2
     sec = t1 - t0
     kms = tm["force_ms"] / max(tm["force_launches"], 1)
-    flop = kernel_flop(shape)
+    flop = convention_flop(shape)
     xflop = executed_flop(shape)
     res = {
         "pairs_per_s": N * N * steps / sec,
@@ -160,8 +185,8 @@ def run_precision(nb, prec, state, steps, warmup, log, pairing="directed"):
         "steps_per_s": steps / sec,
         "kernel_ms": kms,
         "kernel_launches": tm["force_launches"],
-        "kernel_flop": flop,
-        "kernel_tflops": flop / (kms * 1e-3) / 1e12,
+        "convention_flop": flop,
+        "effective_tflops": flop / (kms * 1e-3) / 1e12,
         "executed_flop": xflop,
         "executed_tflops": xflop / (kms * 1e-3) / 1e12,
         "energy_drift": abs(e1 - e0) / abs(e0),
@@ -190,17 +215,34 @@ def traffic_of(traffic, kern):
     return None, None
 
 
-def roofline(prec, res, traffic):
+def roofline(prec, res, traffic, lease):
+    """Roofline object of one leg: achieved = the flop its kernels perform per
+    evaluation / their mean evaluation time from HIP events (frac is a
+    utilisation of the vector peak); effective_tflops = the 60-flop
+    convention over all N^2 directed pairs / the same time."""
     tb, tk = traffic_of(traffic, kname(prec, res))
-    out = {"bound": "valu", "achieved": res["kernel_tflops"], "peak": PEAK[prec],
-           "unit": "TFLOP/s", "frac": res["kernel_tflops"] / PEAK[prec],
+    sym = res["shape"].get("pairing") == "symmetric"
+    out = {"bound": "valu", "achieved": res["executed_tflops"], "peak": PEAK[prec],
+           "unit": "TFLOP/s", "frac": res["executed_tflops"] / PEAK[prec],
            "bound_note": "vector-ALU FLOP roofline (MI355X vector peak: FP64 78.6, FP32 157.3 TFLOP/s): "
                          "the softened r^-3 pair sum is VALU work, HBM-bound by no measure and no MFMA contraction",
-           "traffic": tb, "traffic_kernel": tk, "algorithmic_bytes": 104 * N,
+           "traffic": tb, "traffic_kernel": tk,
+           "traffic_source": (f"{TRAFFIC_FILE}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over "
+                              f"bench.py on {lease or 'an earlier lease'}, not this run") if tb is not None else None,
+           "algorithmic_bytes": 104 * N,
            "kernel": kname(prec, res), "kernel_ms": res["kernel_ms"],
-           "flop_per_launch": res["kernel_flop"],
-           "flop_convention": "60 flop per directed pair, N^2 directed pairs per evaluation"}
-    if res["shape"].get("pairing") == "symmetric":
+           "kernel_ms_covers": ("the centre-of-mass velocity reduction (mixed), the block-diagonal pass and the "
+                                "symmetric kernel" if sym else
+                                "the centre-of-mass velocity reduction (mixed), the source packing and the pair "
+                                "kernel (not the split sums)"),
+           "flop_per_launch": res["executed_flop"],
+           "flop_convention": ("performed work: 73 flop per unordered pair between blocks (60 for acceleration "
+                               "and jerk, 13 for the reaction) + 60 per directed pair inside the blocks" if sym else
+                               "performed work: 60 flop per directed pair, N^2 directed pairs per evaluation"),
+           "effective_tflops": res["effective_tflops"],
+           "effective_note": "60 flop x N^2 directed pairs per evaluation / the same time (a rate; for "
+                             "symmetric pairing it credits pairs the kernel does not evaluate twice)"}
+    if sym:
         b = sym_block(res["shape"])
         fl = SYM_FLUSH_TILES[prec]
         out["traffic_note"] = (f"mostly FP64 atomic adds: the per-tile source totals, about 3 N^2 / {b} per launch "
@@ -208,10 +250,6 @@ def roofline(prec, res, traffic):
                                f"(3 N^2 / {64 * fl}), performed at the memory side (the 8 XCDs' L2s are not "
                                "coherent); besides, each workgroup reads its target block and source blocks "
                                "through L2. algorithmic_bytes = 7 doubles read + 6 written per body")
-        out["executed"] = {"flop_per_launch": res["executed_flop"], "achieved": res["executed_tflops"],
-                           "frac": res["executed_tflops"] / PEAK[prec],
-                           "convention": "73 flop per unordered pair + 60 per directed pair of the "
-                                         "block-diagonal pass"}
     else:
         out["traffic_note"] = ("the source records re-read through L2 by every workgroup, and the j-split "
                                "partial planes written and reduced by a second kernel. "
@@ -288,55 +326,106 @@ def run_adjoint(nb, log):
     return out
 
 
-def _time_force(run, state, budget_s):
-    """One evaluation of run() on the first M bodies, M sized for ~budget_s."""
-    m0 = 16384
-    t = time.perf_counter()
-    run(state[:, :m0])
-    rate = m0 * (m0 - 1) / (time.perf_counter() - t)
-    m = int(np.sqrt(rate * budget_s))
-    m = int(min(max(m, m0), 262144)) // 1024 * 1024
-    t = time.perf_counter()
-    run(state[:, :m])
-    sec = time.perf_counter() - t
-    return m, sec, m * (m - 1) / sec
+def host_info():
+    """CPU model and counts of this host, and the CPU share this process may
+    use (OMP_NUM_THREADS when set -- the GPU box allots each GPU's jobs a
+    share of its cores --, else the CPUs it may run on)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cpus = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = cpus
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or allowed
+    return {"cpu_model": model, "host_cpus": cpus, "cpus_allowed": allowed, "cores_used": min(share, allowed)}
 
 
-def cpu_baseline(state, budget_s=12.0):
+def _mpi_force(exe, state, m, ranks, reps):
+    """Times of `reps` evaluations of the reference's force over the first m
+    bodies by the MPI program exe (oracle/_ref/ref_mpi_force*), `ranks` ranks
+    of one OpenMP thread each."""
+    mpiexec = os.path.join(os.environ.get("MPI_HOME", "/opt/conda"), "bin", "mpiexec")
+    with tempfile.NamedTemporaryFile(suffix=".bin", dir="/tmp") as f:
+        np.ascontiguousarray(state[:, :m]).tofile(f.name)
+        r = subprocess.run([mpiexec, "-n", str(ranks), "-genv", "OMP_NUM_THREADS", "1", exe, f.name, str(m),
+                            str(reps)], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0 or "ref_mpi_force" not in r.stdout:
+        raise RuntimeError(f"{exe} under mpiexec failed ({r.returncode}): {r.stderr[-500:]}")
+    words = r.stdout.split()
+    k = words.index("times")
+    return [float(w) for w in words[k + 1:k + 1 + reps]]
+
+
+def cpu_baseline(state, budget_s=4.0, reps=3):
     """The reference's CPU force_calculation, FP64 (src/kernel.cpp:1164-1224)
     and _F32 (src/kernel.cpp:1101-1163) builds, compiled from its own sources
-    into oracle/_ref (EPS2 = 1e-7: same work per pair), on the first M bodies
-    of the same sphere with OpenMP threads on this host's cores; when that
-    build is absent, the oracle's restatement of the same loops.  M is sized
-    for ~budget_s of CPU time each."""
+    into oracle/_ref (EPS2 = 1e-7: the same work per pair), run the way the
+    reference runs: one MPI rank per core, one OpenMP thread each, cyclic
+    rows of the triangle per rank.  Median of `reps` timings of one
+    evaluation over the first M bodies of the same sphere, M sized for
+    ~budget_s per evaluation.  When mpiexec or the MPI build is unavailable:
+    the same compiled loop in one process with OpenMP threads, then the
+    oracle's restatement (kind "port")."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle, Reference
-    cores = max(1, min(16, os.cpu_count() or 1))
-    runs = {}
-    try:
+    info = host_info()
+    cores = info["cores_used"]
+    out, how = {}, None
+    for prec, tag in (("fp64", "64"), ("f32", "32")):
+        exe = os.path.join(ROOT, "oracle", "_ref", f"ref_mpi_force{tag}")
+        try:
+            t0 = _mpi_force(exe, state, 16384, cores, 1)[0]
+            m = int(min(max(np.sqrt(16384 * 16383 / t0 * budget_s), 16384), N)) // 1024 * 1024
+            ts = _mpi_force(exe, state, m, cores, reps)
+            out[prec] = (m, ts)
+            how = "mpi"
+        except (RuntimeError, OSError, ValueError, subprocess.TimeoutExpired) as e:
+            print(f"[bench] MPI baseline unavailable ({e}); timing in one process", file=sys.stderr, flush=True)
+            how = None
+            break
+    kind = "reference"
+    if how is None:
+        from oracle import Oracle, Reference
+        try:
+            runs = {p: (lambda st, R=Reference(p): R.force(st, nthreads=cores)) for p in ("fp64", "f32")}
+            how = "openmp"
+        except (RuntimeError, OSError):
+            O = Oracle()
+            runs = {p: (lambda st, p=p: O.force(st, EPS * EPS, p, nthreads=cores)) for p in ("fp64", "f32")}
+            how, kind = "openmp", "port"
         for prec in ("fp64", "f32"):
-            R = Reference(prec)
-            runs[prec] = (lambda s, R=R: R.force(s, nthreads=cores))
-        kind = "reference"
-        what = "the reference's own {} force_calculation (compiled from its sources, EPS2=1e-7)"
-    except (RuntimeError, OSError):
-        O = Oracle()
-        for prec in ("fp64", "f32"):
-            runs[prec] = (lambda s, p=prec: O.force(s, EPS * EPS, p, nthreads=cores))
-        kind = "port"
-        what = "the oracle's restatement of the reference {} loop (eps=1e-3)"
-    out = {}
+            t = time.perf_counter()
+            runs[prec](state[:, :16384])
+            m = int(min(max(np.sqrt(16384 * 16383 / (time.perf_counter() - t) * budget_s), 16384), N)) // 1024 * 1024
+            ts = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                runs[prec](state[:, :m])
+                ts.append(time.perf_counter() - t)
+            out[prec] = (m, ts)
+    res = {}
     for prec in ("fp64", "f32"):
-        m, sec, rate = _time_force(runs[prec], state, budget_s)
-        out[prec] = (m, sec, rate)
-    m, sec, rate = out["fp64"]
-    m32, sec32, rate32 = out["f32"]
-    return {"value": rate, "unit": "pair-interactions/s", "cores": cores, "kind": kind,
-            "sample": f"one evaluation of {what.format('FP64')}, triangle order, over the first {m} bodies of "
-                      f"the N=1048576 seed-3 Plummer sphere, {cores} OpenMP threads (one process), {sec:.1f} s; "
-                      f"rate = M(M-1)/t directed pairs",
-            "f32": {"value": rate32, "unit": "pair-interactions/s", "cores": cores, "kind": kind,
-                    "sample": f"the same with {what.format('_F32')}, first {m32} bodies, {sec32:.1f} s"}}
+        m, ts = out[prec]
+        med = float(np.median(ts))
+        res[prec] = {"value": m * (m - 1) / med, "unit": "pair-interactions/s", "cores": cores, "kind": kind,
+                     "bodies": m, "times_s": ts, "median_s": med}
+    layout = (f"{cores} MPI ranks x 1 OpenMP thread (mpiexec)" if how == "mpi"
+              else f"one process, {cores} OpenMP threads")
+    src = ("the reference's own force_calculation compiled from its sources (EPS2=1e-7)" if kind == "reference"
+           else "the oracle's restatement of the reference loop (eps=1e-3)")
+    r64, r32 = res["fp64"], res["f32"]
+    return {"value": r64["value"], "unit": "pair-interactions/s", "cores": cores, "kind": kind,
+            "sample": f"{src}, FP64 build: median of {len(r64['times_s'])} evaluations over the first "
+                      f"{r64['bodies']} bodies of the N=1048576 seed-3 Plummer sphere ({r64['median_s']:.2f} s each), "
+                      f"triangle order, {layout}; rate = M(M-1)/t directed pairs",
+            "layout": layout, "cpu_model": info["cpu_model"], "host_cpus": info["host_cpus"],
+            "cpus_allowed": info["cpus_allowed"], "fp64": r64, "f32": r32}
 
 
 def main():
@@ -383,10 +472,14 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and rank == 0:
         cpu = cpu_baseline(state)
-        log(f"cpu baseline {cpu['value']:.3e} pairs/s (FP64), {cpu['f32']['value']:.3e} (_F32) "
-            f"on {cpu['cores']} threads, {cpu['kind']}")
+        log(f"cpu baseline {cpu['value']:.3e} pairs/s (FP64), {cpu['f32']['value']:.3e} (_F32): {cpu['layout']}, "
+            f"{cpu['kind']}, {cpu['cpu_model']}, {cpu['host_cpus']} host CPUs")
 
-    traffic = measured_traffic()
+    traffic, lease = measured_traffic()
+    ratio_dd = rmd["pairs_per_s"] / r64["pairs_per_s"]
+    ratio_ss = rmx["pairs_per_s"] / r64s["pairs_per_s"]
+    ratio_sd = rmx["pairs_per_s"] / r64["pairs_per_s"]
+    rf64 = roofline("fp64", r64, traffic, lease)
     out = {
         "metric": "pair-interactions/s at N=1M: mixed vs FP64 GPU @ median rel acc err <=1e-6",
         "value": rmx["pairs_per_s"],
@@ -406,25 +499,32 @@ def main():
         "mixed_median_rel_acc_err": med,
         "mixed_max_rel_acc_err": mx,
         "fp64_pairs_per_s": r64["pairs_per_s"],
-        "mixed_over_fp64": {"symmetric_over_symmetric": rmx["pairs_per_s"] / r64s["pairs_per_s"],
-                            "directed_over_directed": rmd["pairs_per_s"] / r64["pairs_per_s"],
-                            "symmetric_over_directed_dropin": rmx["pairs_per_s"] / r64["pairs_per_s"]},
+        "mixed_over_fp64": {"symmetric_over_symmetric": ratio_ss,
+                            "directed_over_directed": ratio_dd,
+                            "symmetric_over_directed_dropin": ratio_sd},
+        "target": {"mixed_over_fp64_required": 2.5,
+                   "mixed_over_fp64_like_for_like": {"directed": ratio_dd, "symmetric": ratio_ss},
+                   "mixed_symmetric_over_fp64_dropin": ratio_sd,
+                   "ratio_met": max(ratio_dd, ratio_ss, ratio_sd) >= 2.5,
+                   "fp64_frac_required": 0.5, "fp64_frac": rf64["frac"], "fp64_frac_met": rf64["frac"] >= 0.5,
+                   "accuracy_met": med <= 1e-6,
+                   "drift_met": rmx["energy_drift"] <= 10 * r64["energy_drift"]},
         "mixed_pairing": rmx["shape"]["pairing"],
         "fp64_symmetric": {"pairs_per_s": r64s["pairs_per_s"], "over_fp64_directed": r64s["pairs_per_s"] / r64["pairs_per_s"],
                            "max_rel_err_vs_directed": float(err64s.max()), "energy_drift": r64s["energy_drift"],
-                           "launch_shape": r64s["shape"], "roofline": roofline("fp64", r64s, traffic)},
+                           "launch_shape": r64s["shape"], "roofline": roofline("fp64", r64s, traffic, lease)},
         "mixed_directed": {"pairs_per_s": rmd["pairs_per_s"], "over_fp64": rmd["pairs_per_s"] / r64["pairs_per_s"],
                            "median_rel_acc_err": float(np.median(errd)), "max_rel_acc_err": float(errd.max()),
                            "energy_drift": rmd["energy_drift"], "launch_shape": rmd["shape"],
-                           "roofline": roofline("mixed", rmd, traffic)},
+                           "roofline": roofline("mixed", rmd, traffic, lease)},
         "steps_per_s": {"fp64": r64["steps_per_s"], "fp64_symmetric": r64s["steps_per_s"],
                         "mixed": rmx["steps_per_s"], "mixed_directed": rmd["steps_per_s"]},
         "energy_drift": {"fp64": r64["energy_drift"], "fp64_symmetric": r64s["energy_drift"],
                          "mixed": rmx["energy_drift"], "mixed_directed": rmd["energy_drift"]},
         "launch_shape": rmx["shape"],
         "dropin_host_arrays": {"fp64": r64["dropin"], "mixed": rmx["dropin"]},
-        "roofline": roofline("mixed", rmx, traffic),
-        "roofline_fp64": roofline("fp64", r64, traffic),
+        "roofline": roofline("mixed", rmx, traffic, lease),
+        "roofline_fp64": rf64,
         "cpu_baseline": cpu,
         "ensemble": ens,
         "adjoint": adj,

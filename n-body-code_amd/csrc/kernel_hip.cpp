@@ -40,6 +40,8 @@ nbh_ctx* force_backend_context()
     return g_ctx;
 }
 
+double force_backend_eps2() { return g_fb.eps2; }
+
 void force_backend_shutdown()
 {
     if (g_ctx) nbh_destroy(g_ctx);

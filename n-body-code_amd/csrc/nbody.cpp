@@ -186,6 +186,11 @@ void NBody::timestepping()
     force_calculation(h_par_x, h_par_y, h_par_z, h_par_w, h_vel_x, h_vel_y, h_vel_z, h_acc1_x,
                       h_acc1_y, h_acc1_z, h_a1dot_x, h_a1dot_y, h_a1dot_z, sys);
     sys.t_eval = sys.kernel_time;
+#if defined(_DEBUG)
+    // the reference's debug build checks every evaluation (src/nbody.cpp:248-252)
+    test(h_par_x, h_par_y, h_par_z, h_par_w, h_vel_x, h_vel_y, h_vel_z, h_acc1_x, h_acc1_y, h_acc1_z,
+         h_a1dot_x, h_a1dot_y, h_a1dot_z, N);
+#endif
 
     snapandcrackle();
     t0 = now();
@@ -215,6 +220,10 @@ void NBody::cycle(int ncycle)
     force_calculation(h_par_x, h_par_y, h_par_z, h_par_w, h_vel_x, h_vel_y, h_vel_z, h_acc_x,
                       h_acc_y, h_acc_z, h_adot_x, h_adot_y, h_adot_z, sys);
     sys.t_eval_init = sys.kernel_time;
+#if defined(_DEBUG)
+    test(h_par_x, h_par_y, h_par_z, h_par_w, h_vel_x, h_vel_y, h_vel_z, h_acc_x, h_acc_y, h_acc_z, h_adot_x,
+         h_adot_y, h_adot_z, N);  // src/nbody.cpp:326-330
+#endif
     dt = fixed_dt > 0.0 ? fixed_dt : initialdt();
     T = compute_kinetic(h_par_w, h_vel_x, h_vel_y, h_vel_z, N);
     U = compute_potential(h_par_x, h_par_y, h_par_z, h_par_w, N, compat_energy);

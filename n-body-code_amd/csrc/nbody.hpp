@@ -74,6 +74,19 @@ double compute_potential(double* h_par_x, double* h_par_y, double* h_par_z, doub
                          bool compat = true);
 double estimate_err(double En, double E0);
 
+// The rest of the reference's host utilities (include/utils.h:29,55-63).
+double timer();                 // wall-clock seconds
+double InvSqrt2D(double x);     // the reference's fast 1/sqrt, bitwise (src/utils.cpp:155-165)
+float InvSqrt2(float x);        // its FP32 form (src/utils.cpp:167-175)
+double virial_ratio(double* h_par_x, double* h_par_y, double* h_par_z, double* h_par_w, double* h_vel_x,
+                    double* h_vel_y, double* h_vel_z, int N);  // -2T/U, reference energies
+// _DEBUG brute-force check of a force evaluation (src/utils.cpp:187-275).
+void test(double* h_par_x, double* h_par_y, double* h_par_z, double* h_par_w, double* h_vel_x,
+          double* h_vel_y, double* h_vel_z, double* h_acc_x, double* h_acc_y, double* h_acc_z,
+          double* h_adot_x, double* h_adot_y, double* h_adot_z, int N);
+// Softening of the force backend (the eps2 force_calculation uses).
+double force_backend_eps2();
+
 class NBody {
 public:
     double alpha;

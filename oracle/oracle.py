@@ -173,6 +173,8 @@ class Reference:
         self.lib = L
         L.ref_invsqrt2d.restype = C.c_double
         L.ref_invsqrt2d.argtypes = [C.c_double]
+        L.ref_invsqrt2.restype = C.c_float
+        L.ref_invsqrt2.argtypes = [C.c_float]
         L.ref_eps2.restype = C.c_double
         L.ref_force.restype = None
         L.ref_force.argtypes = [C.c_int] + [_D] * 13
@@ -193,6 +195,9 @@ class Reference:
 
     def invsqrt2d(self, x):
         return self.lib.ref_invsqrt2d(float(x))
+
+    def invsqrt2(self, x):
+        return self.lib.ref_invsqrt2(float(x))
 
     def force(self, state, nthreads=1):
         """force_calculation of the compiled reference; nthreads > 1 runs its

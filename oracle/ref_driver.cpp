@@ -36,6 +36,8 @@ extern "C" {
 
 double ref_invsqrt2d(double x) { return InvSqrt2D(x); }
 
+float ref_invsqrt2(float x) { return InvSqrt2(x); }
+
 double ref_eps2() { return EPS2; }
 
 // force_calculation (include/kernel.h:15-19) on one rank, with nthreads
