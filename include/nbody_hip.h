@@ -286,13 +286,19 @@ int nbh_ens_download(nbh_ensemble *ens, double *x, double *y, double *z,
                      double *jy, double *jz);
 /* Per-system kinetic and softened potential energy (FP64), arrays [nsys]. */
 int nbh_ens_energy(nbh_ensemble *ens, double *kinetic, double *potential);
-/* Times the candidate (tile in {64,128,256}) x (jsplit in {1,2,4,8}) for
+/* Times the candidate (tile in {0,64,128,256}) x (jsplit in {1,2,4,8}) for
  * every populated bucket on the device and keeps the fastest, then one or
  * two targets per lane for the whole work list.  Outputs [NBH_ENS_BUCKETS]
  * each, may be NULL; ms = best force time of the bucket alone (0 for empty
  * buckets). */
 int nbh_ens_autotune(nbh_ensemble *ens, int *tiles, int *jsplits, double *ms);
-/* Sets the per-bucket tile (64, 128 or 256) and jsplit (>= 1) directly. */
+/* Sets the per-bucket tile (64, 128 or 256) and jsplit (>= 1) directly.
+ * Tile 0 evaluates the bucket's systems (those not taken by symmetric
+ * pairing) with the scalar-load kernel instead: one four-wave workgroup per
+ * 256 targets of a system against all its sources, read from records packed
+ * once per evaluation at wave-uniform indices (scalar loads, no LDS); its
+ * jsplit is ignored.  Either path sums each target's sources in the same
+ * order, so with jsplit 1 the forces are bitwise the same. */
 int nbh_ens_set_tuning(nbh_ensemble *ens, const int *tiles, const int *jsplits);
 /* Sets the targets per lane (1 or 2) of every work item. */
 int nbh_ens_set_targets_per_lane(nbh_ensemble *ens, int ipl);

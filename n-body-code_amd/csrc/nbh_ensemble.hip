@@ -31,7 +31,9 @@ namespace {
 constexpr int kLanes = 64;
 constexpr int kMaxTargets = 128;  // targets per item: 64 per lane-set, up to two sets
 const int kBucketEdge[NBH_ENS_BUCKETS] = {64, 256, 1024, INT_MAX};
-const int kTiles[3] = {64, 128, 256};
+// LDS tile lengths of the persistent kernel; 0 selects the scalar-load
+// kernel (k_ens_big) for a bucket's systems instead
+const int kTiles[4] = {0, 64, 128, 256};
 const int kSplits[4] = {1, 2, 4, 8};
 
 struct EnsSys {
@@ -48,6 +50,12 @@ struct EnsItem {
     int tile;    // LDS tile length for this item
     int slot;    // partial-sum slot, or -1: write the forces directly
     int ipl;     // targets per lane of the list: lane t holds i0 + t and, if 2, i0 + 64 + t
+};
+
+// One workgroup of the scalar-load kernel: up to 256 consecutive targets of
+// system `sys` against its sources [j0, j1) (global indices).
+struct EnsBig {
+    int sys, i0, icount, j0, j1;
 };
 
 struct EnsReduce {
@@ -275,6 +283,83 @@ __global__ __launch_bounds__(kLanes) void k_ens_vref(State s, const EnsSys* __re
     }
 }
 
+// Source records of the systems the scalar-load kernel covers (list), one
+// workgroup per system: record j of body j, velocities less the system's
+// reference velocity in mixed precision.
+constexpr int kBigBlock = 256;
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBigBlock) void k_ens_pack(State s, const EnsSys* __restrict__ sys,
+                                                      const int* __restrict__ list, const double* __restrict__ vref,
+                                                      void* __restrict__ rec)
+{
+    const int k = list[blockIdx.x];
+    const EnsSys q = sys[k];
+    if constexpr (MIXED) {
+        const VRef vr = load_vref(vref + 3 * k);
+        for (int j = q.off + (int)threadIdx.x; j < q.off + q.n; j += kBigBlock)
+            static_cast<JBMX*>(rec)[j] = load_jbmx(s, j, vr);
+    } else {
+        for (int j = q.off + (int)threadIdx.x; j < q.off + q.n; j += kBigBlock)
+            static_cast<JB64*>(rec)[j] = load_jb64(s, j);
+    }
+}
+
+// Scalar-load kernel for the large systems: the single-system directed
+// kernel's structure (one target per lane, four waves per workgroup, the
+// source record at a wave-uniform index so that it is read with scalar
+// loads into SGPRs, no LDS), here over one system's targets and sources per
+// workgroup.  Every wave of a workgroup streams the same records, which the
+// scalar cache then serves to all four.  Each target sums its sources in
+// order from j0, FP32 runs of kFold folded into the two-sum (hi, lo) totals
+// in mixed precision: the same sums, bit for bit, as the persistent kernel
+// over the same source range.
+template <bool MIXED, bool MASK>
+__global__ __launch_bounds__(kBigBlock) void k_ens_big(State s, Forces out, const EnsSys* __restrict__ sys,
+                                                     const EnsBig* __restrict__ items, const void* __restrict__ rec,
+                                                     const double* __restrict__ vref)
+{
+    const EnsBig it = items[blockIdx.x];
+    const int t = threadIdx.x;
+    const int i = it.i0 + (t < it.icount ? t : 0);
+    const double eps2 = sys[it.sys].eps2;
+    const double xi = s.x[i], yi = s.y[i], zi = s.z[i];
+    double f[6];
+    if constexpr (MIXED) {
+        const VRef vr = load_vref(vref + 3 * it.sys);
+        const float vxi = (float)(s.vx[i] - vr.x), vyi = (float)(s.vy[i] - vr.y), vzi = (float)(s.vz[i] - vr.z);
+        const float e2 = (float)eps2;
+        const JBMX* __restrict__ r = static_cast<const JBMX*>(rec);
+        float hi[6] = {0, 0, 0, 0, 0, 0}, lo[6] = {0, 0, 0, 0, 0, 0};
+        for (int j0 = it.j0; j0 < it.j1; j0 += kFold) {
+            float acc[6] = {0, 0, 0, 0, 0, 0};
+            if (j0 + kFold <= it.j1) {
+#pragma unroll 8
+                for (int j = j0; j < j0 + kFold; ++j)
+                    pair_mx<MASK>(r[j], j, i, xi, yi, zi, vxi, vyi, vzi, e2, acc);
+            } else {
+                for (int j = j0; j < it.j1; ++j) pair_mx<MASK>(r[j], j, i, xi, yi, zi, vxi, vyi, vzi, e2, acc);
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) two_sum_fold(hi[c], lo[c], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) f[c] = (double)hi[c] + (double)lo[c];
+    } else {
+        const double vxi = s.vx[i], vyi = s.vy[i], vzi = s.vz[i];
+        const JB64* __restrict__ r = static_cast<const JB64*>(rec);
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+        for (int j = it.j0; j < it.j1; ++j) pair_f64<MASK>(r[j], j, i, xi, yi, zi, vxi, vyi, vzi, eps2, acc);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) f[c] = acc[c];
+    }
+    if (t < it.icount) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) out.c[c][it.i0 + t] = f[c];
+    }
+}
+
 // Per-system kinetic and softened potential energy (FP64), one workgroup per
 // system, block sums in a fixed order.
 constexpr int kEnBlock = 256;
@@ -365,7 +450,8 @@ struct nbh_ensemble {
 
     // device
     DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot, symitems, vref;
-    int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0, nsym = 0;
+    DevBuf bigitems, biglist, rec;  // scalar-load kernel: items, its systems, packed records
+    int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0, nsym = 0, nbig = 0, nbigsys = 0;
     int list_ipl = 1;  // targets per lane of the uploaded work list
     bool uploaded = false, dt_set = false;
     hipGraphExec_t gexec = nullptr;
@@ -374,6 +460,7 @@ struct nbh_ensemble {
 
 namespace {
 
+using nbh::EnsBig;
 using nbh::EnsItem;
 using nbh::EnsReduce;
 
@@ -390,14 +477,31 @@ void drop_graph(nbh_ensemble* e)
 // items for its diagonal blocks only (each target against its own block,
 // written directly) and symmetric items for its block pairs, which the
 // symmetric kernel adds afterwards.
+// A bucket with tile 0 puts its (directed) systems on the scalar-load
+// kernel instead: one item per 256 targets, the whole system as sources.
+struct WorkList {
+    std::vector<EnsItem> items;
+    std::vector<EnsReduce> reds;
+    std::vector<nbh::SymItem> sym;
+    std::vector<EnsBig> big;
+    std::vector<int> bigsys;
+    int nslots = 0;
+    int maxtile() const
+    {
+        int t = 64;
+        for (const auto& it : items) t = std::max(t, it.tile);
+        return t;
+    }
+};
+
 void build_items(const nbh_ensemble* e, const int* tile, const int* split, int ipl,
-                 const std::vector<char>& use, std::vector<EnsItem>& items,
-                 std::vector<EnsReduce>& reds, int& nslots, std::vector<nbh::SymItem>& sym)
+                 const std::vector<char>& use, WorkList& w)
 {
-    items.clear();
-    reds.clear();
-    sym.clear();
-    nslots = 0;
+    w = WorkList{};
+    std::vector<EnsItem>& items = w.items;
+    std::vector<EnsReduce>& reds = w.reds;
+    std::vector<nbh::SymItem>& sym = w.sym;
+    int& nslots = w.nslots;
     const int prec = e->ctx->cfg.precision;
     const int SB = nbh::ens_sym_block(prec);
     for (int k = 0; k < e->nsys; ++k) {
@@ -408,14 +512,23 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
         const int Q = ipl;
         const int per = nbh::kLanes * Q;
         if (e->sym && nbh::ens_sym_supported(prec, n)) {
+            // diagonal blocks on the persistent LDS kernel (a bucket set to
+            // the scalar-load kernel, tile 0, takes tiles of 256 here)
+            const int TD = T > 0 ? T : 256;
             const int nb = (n + SB - 1) / SB, nr = n / SB;
             for (int P = 0; P < nb; ++P) {
                 const int b0 = P * SB, b1 = std::min(n, b0 + SB);
                 for (int ib = b0; ib < b1; ib += per)
-                    items.push_back({k, off + ib, std::min(per, b1 - ib), off + b0, off + b1, T, -1, Q});
+                    items.push_back({k, off + ib, std::min(per, b1 - ib), off + b0, off + b1, TD, -1, Q});
                 const int len = P < nr ? nbh::sym_list_len(nr, P, nb > nr) : 0;
                 if (len >= 1) sym.push_back({off, n, P, 0, len - 1, k, e->eps2[k]});
             }
+            continue;
+        }
+        if (T == 0) {
+            for (int ib = 0; ib < n; ib += nbh::kBigBlock)
+                w.big.push_back({k, off + ib, std::min(nbh::kBigBlock, n - ib), off, off + n});
+            w.bigsys.push_back(k);
             continue;
         }
         int S = std::min(split[b], (n + T - 1) / T);
@@ -449,12 +562,41 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
     // largest source range first; stable keeps system order among equals
     std::stable_sort(items.begin(), items.end(),
                      [](const EnsItem& a, const EnsItem& b) { return (a.j1 - a.j0) > (b.j1 - b.j0); });
+    std::stable_sort(w.big.begin(), w.big.end(),
+                     [](const EnsBig& a, const EnsBig& b) { return (a.j1 - a.j0) > (b.j1 - b.j0); });
 }
 
-int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::vector<EnsReduce>& reds,
-                 int nslots, int maxtile, int ipl, const std::vector<nbh::SymItem>& sym)
+int upload_items(nbh_ensemble* e, const WorkList& w, int ipl)
 {
     nbh_ctx* c = e->ctx;
+    const std::vector<EnsItem>& items = w.items;
+    const std::vector<EnsReduce>& reds = w.reds;
+    const std::vector<nbh::SymItem>& sym = w.sym;
+    const int nslots = w.nslots;
+    const int maxtile = w.maxtile();
+    // the persistent kernel steps through an item's sources a tile at a time:
+    // every item must have a tile and a non-empty range within its system
+    for (const auto& it : items)
+        if (!(it.tile == 64 || it.tile == 128 || it.tile == 256) || it.j1 <= it.j0 || it.icount < 1 ||
+            it.icount > nbh::kLanes * it.ipl)
+            return fail(NBH_ERR_STATE, "ensemble work item of system %d has tile %d, sources [%d, %d), %d targets",
+                        it.sys, it.tile, it.j0, it.j1, it.icount);
+    for (const auto& b : w.big)
+        if (b.j1 <= b.j0 || b.icount < 1 || b.icount > nbh::kBigBlock)
+            return fail(NBH_ERR_STATE, "scalar-load item of system %d has sources [%d, %d), %d targets", b.sys,
+                        b.j0, b.j1, b.icount);
+    if (!w.big.empty()) {
+        if (e->bigitems.reserve(sizeof(EnsBig) * w.big.size()) != hipSuccess ||
+            e->biglist.reserve(sizeof(int) * w.bigsys.size()) != hipSuccess ||
+            e->rec.reserve(sizeof(nbh::JB64) * (size_t)e->total) != hipSuccess)
+            return fail(NBH_ERR_ALLOC, "ensemble scalar-load buffers allocation failed");
+        NBH_HIP(hipMemcpyAsync(e->bigitems.p, w.big.data(), sizeof(EnsBig) * w.big.size(), hipMemcpyHostToDevice,
+                               c->stream));
+        NBH_HIP(hipMemcpyAsync(e->biglist.p, w.bigsys.data(), sizeof(int) * w.bigsys.size(),
+                               hipMemcpyHostToDevice, c->stream));
+    }
+    e->nbig = (int)w.big.size();
+    e->nbigsys = (int)w.bigsys.size();
     if (e->symitems.reserve(sizeof(nbh::SymItem) * std::max<size_t>(sym.size(), 1)) != hipSuccess ||
         e->items.reserve(sizeof(EnsItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
         e->reduces.reserve(sizeof(EnsReduce) * std::max<size_t>(reds.size(), 1)) != hipSuccess ||
@@ -487,15 +629,10 @@ int upload_items(nbh_ensemble* e, const std::vector<EnsItem>& items, const std::
 
 int rebuild(nbh_ensemble* e)
 {
-    std::vector<EnsItem> items;
-    std::vector<EnsReduce> reds;
-    std::vector<nbh::SymItem> sym;
-    int nslots = 0;
-    build_items(e, e->tile, e->split, e->ipl, {}, items, reds, nslots, sym);
-    int maxt = 64;
-    for (const auto& it : items) maxt = std::max(maxt, it.tile);
+    WorkList w;
+    build_items(e, e->tile, e->split, e->ipl, {}, w);
     drop_graph(e);
-    return upload_items(e, items, reds, nslots, maxt, e->ipl, sym);
+    return upload_items(e, w, e->ipl);
 }
 
 // Force evaluation of the current work list: state -> out.
@@ -511,13 +648,6 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
     a.counter = static_cast<unsigned*>(e->counter.p);
     a.partial = e->partial.d();
     a.vref = e->vref.d();
-    nbh::k_zero_u32<<<1, 64, 0, c->stream>>>(a.counter);
-    NBH_HIP(hipGetLastError());
-    const bool mixed = c->cfg.precision == NBH_MIXED;
-    if (mixed) {
-        nbh::k_ens_vref<<<e->nsys, nbh::kLanes, 0, c->stream>>>(a.s, a.sys, e->nsys, e->vref.d());
-        NBH_HIP(hipGetLastError());
-    }
     nbh::EventGuard ev;
     const bool timed = c->profiling && allow_events;
     if (timed) {
@@ -525,26 +655,48 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
         if (rc) return rc;
         NBH_HIP(hipEventRecord(ev.ep.a, c->stream));
     }
-    const void* k = nbh::pick_ens_kernel(mixed, e->maxtile, e->any_mask[mixed], e->list_ipl);
+    nbh::k_zero_u32<<<1, 64, 0, c->stream>>>(a.counter);
+    NBH_HIP(hipGetLastError());
+    const bool mixed = c->cfg.precision == NBH_MIXED;
+    const bool mask = e->any_mask[mixed];
+    if (mixed) {
+        nbh::k_ens_vref<<<e->nsys, nbh::kLanes, 0, c->stream>>>(a.s, a.sys, e->nsys, e->vref.d());
+        NBH_HIP(hipGetLastError());
+    }
+    if (e->nbig > 0) {  // the large systems' source records, then their scalar-load kernel
+        const int* list = static_cast<const int*>(e->biglist.p);
+        if (mixed)
+            nbh::k_ens_pack<true><<<e->nbigsys, nbh::kBigBlock, 0, c->stream>>>(a.s, a.sys, list, a.vref, e->rec.p);
+        else
+            nbh::k_ens_pack<false><<<e->nbigsys, nbh::kBigBlock, 0, c->stream>>>(a.s, a.sys, list, a.vref, e->rec.p);
+        NBH_HIP(hipGetLastError());
+        const auto* big = static_cast<const EnsBig*>(e->bigitems.p);
+        const dim3 g(e->nbig), b(nbh::kBigBlock);
+        if (mixed && mask)
+            nbh::k_ens_big<true, true><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
+        else if (mixed)
+            nbh::k_ens_big<true, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
+        else if (mask)
+            nbh::k_ens_big<false, true><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
+        else
+            nbh::k_ens_big<false, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
+        NBH_HIP(hipGetLastError());
+    }
+    const void* k = nbh::pick_ens_kernel(mixed, e->maxtile, mask, e->list_ipl);
     if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d", e->maxtile);
     void* args[] = {&a};
     NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, c->stream));
-    if (timed && e->nsym == 0) {
-        NBH_HIP(hipEventRecord(ev.ep.b, c->stream));
-        ev.pending();
-    }
     if (e->nreduce > 0) {
         nbh::k_ens_reduce<<<e->nreduce, nbh::kLanes, 0, c->stream>>>(static_cast<const EnsReduce*>(e->reduces.p),
                                                                       e->nreduce, e->partial.d(), out);
         NBH_HIP(hipGetLastError());
     }
-    if (e->nsym > 0) {
+    if (e->nsym > 0)
         NBH_HIP(nbh::launch_ens_sym(c->cfg.precision, a.s, out, static_cast<const nbh::SymItem*>(e->symitems.p),
                                     e->nsym, mixed ? a.vref : nullptr, c->stream));
-        if (timed) {
-            NBH_HIP(hipEventRecord(ev.ep.b, c->stream));
-            ev.pending();
-        }
+    if (timed) {
+        NBH_HIP(hipEventRecord(ev.ep.b, c->stream));
+        ev.pending();
     }
     return NBH_OK;
 }
@@ -632,7 +784,7 @@ int nbh_ens_destroy(nbh_ensemble* e)
     (void)hipStreamSynchronize(e->ctx->stream);
     drop_graph(e);
     for (DevBuf* b : {&e->sys, &e->state, &e->f0, &e->f1, &e->items, &e->reduces, &e->partial, &e->counter,
-                      &e->dt, &e->ekin, &e->epot, &e->symitems, &e->vref})
+                      &e->dt, &e->ekin, &e->epot, &e->symitems, &e->vref, &e->bigitems, &e->biglist, &e->rec})
         b->release();
     delete e;
     return NBH_OK;
@@ -757,8 +909,9 @@ int nbh_ens_set_tuning(nbh_ensemble* e, const int* tiles, const int* jsplits)
     if (!e) return fail(NBH_ERR_ARG, "ensemble is NULL");
     if (!tiles || !jsplits) return fail(NBH_ERR_ARG, "tiles/jsplits are NULL");
     for (int b = 0; b < NBH_ENS_BUCKETS; ++b) {
-        if (tiles[b] != 64 && tiles[b] != 128 && tiles[b] != 256)
-            return fail(NBH_ERR_ARG, "bucket %d: tile must be 64, 128 or 256, got %d", b, tiles[b]);
+        if (tiles[b] != 0 && tiles[b] != 64 && tiles[b] != 128 && tiles[b] != 256)
+            return fail(NBH_ERR_ARG, "bucket %d: tile must be 0 (scalar-load kernel), 64, 128 or 256, got %d", b,
+                        tiles[b]);
         if (jsplits[b] < 1) return fail(NBH_ERR_ARG, "bucket %d: jsplit must be >= 1, got %d", b, jsplits[b]);
     }
     NBH_HIP(hipSetDevice(e->ctx->cfg.device));
@@ -807,18 +960,16 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
         int bt = e->tile[bk], bs = e->split[bk];
         for (int T : nbh::kTiles) {
             for (int S : nbh::kSplits) {
+                if (T == 0 && S > 1) break;  // the scalar-load kernel takes whole systems
                 int tl[NBH_ENS_BUCKETS], sp[NBH_ENS_BUCKETS];
                 std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, tl);
                 std::copy(e->split, e->split + NBH_ENS_BUCKETS, sp);
                 tl[bk] = T;
                 sp[bk] = S;
-                std::vector<EnsItem> items;
-                std::vector<EnsReduce> reds;
-                std::vector<nbh::SymItem> sym;
-                int nslots = 0;
-                build_items(e, tl, sp, e->ipl, use, items, reds, nslots, sym);
+                WorkList w;
+                build_items(e, tl, sp, e->ipl, use, w);
                 double ms = 0.0;
-                if ((rc = upload_items(e, items, reds, nslots, T, e->ipl, sym)) || (rc = time_list(&ms))) break;
+                if ((rc = upload_items(e, w, e->ipl)) || (rc = time_list(&ms))) break;
                 if (ms < best) {
                     best = ms;
                     bt = T;

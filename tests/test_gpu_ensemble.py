@@ -313,3 +313,40 @@ def test_mixed_systems_moving_apart(nb, oracle, symset, pairing):
             f"jerk {meds[:, 1].max():.2e}")
     assert np.median(meds[:, 0]) <= 1e-6 and np.median(meds[:, 1]) <= 1e-6
     assert meds[:, 0].max() <= 1e-5 and meds[:, 1].max() <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_scalar_load_kernel_bitwise_equals_lds_tiles(nb, oracle, precision):
+    """Tile 0 puts a bucket's systems on the scalar-load kernel (256 targets
+    of one system per four-wave workgroup, sources read with scalar loads):
+    the same sums, bit for bit, as the persistent LDS-tile kernel with one
+    split, for the large bucket alone and for every bucket; and against the
+    oracle."""
+    counts = np.array([1, 2, 63, 64, 65, 300, 1025, 1100, 2048, 2049, 3001, 4096], dtype=np.int32)
+    eps2 = (1.0 / (4.0 * counts.astype(float) ** (1 / 3))) ** 2
+    state = nb.ensemble_state(counts, 29)
+    out = {}
+    for name, tiles in (("lds", [256] * 4), ("large", [256, 256, 256, 0]), ("all", [0] * 4)):
+        with nb.Backend(precision) as b, nb.Ensemble(b, counts, eps2) as e:
+            e.set_tuning(tiles, [1] * 4)
+            e.upload(state)
+            out[name] = e.download()[1]
+            if name == "large":
+                check_systems(e, out[name], state, oracle, eps2, mixed=(precision == "mixed"))
+    _banner(f"{precision} scalar-load ensemble kernel vs LDS tiles")
+    assert np.array_equal(out["large"], out["lds"])
+    assert np.array_equal(out["all"], out["lds"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_symmetric_context_with_scalar_load_buckets(nb, oracle, symset, precision):
+    """Regression: with symmetric pairing, a bucket set to the scalar-load
+    kernel (tile 0) still sends its symmetric systems' diagonal blocks to the
+    persistent LDS kernel, which needs a tile length (it takes 256); the
+    forces match the oracle."""
+    counts, eps2, state = symset
+    with nb.Backend(precision, pairing="symmetric") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.set_tuning([0, 0, 0, 0], [1, 1, 1, 1])
+        e.upload(state)
+        _, f = e.download()
+        check_systems(e, f, state, oracle, eps2, mixed=(precision == "mixed"))
