@@ -930,6 +930,11 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
     NBH_HIP(hipEventCreate(&a));
     NBH_HIP(hipEventCreate(&b));
     double best_ms[NBH_ENS_BUCKETS] = {0, 0, 0, 0};
+    int lds_tile[NBH_ENS_BUCKETS], lds_split[NBH_ENS_BUCKETS];  // each bucket's best LDS-tile shape
+    std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, lds_tile);
+    std::copy(e->split, e->split + NBH_ENS_BUCKETS, lds_split);
+    for (int b = 0; b < NBH_ENS_BUCKETS; ++b)
+        if (lds_tile[b] == 0) lds_tile[b] = 256, lds_split[b] = 1;
     int rc = NBH_OK;
     const bool prof = c->profiling;
     c->profiling = false;
@@ -956,7 +961,7 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
         for (int k = 0; k < e->nsys; ++k)
             if (nbh::bucket_of(e->cnt[k]) == bk) use[k] = 1, any = true;
         if (!any) continue;
-        double best = 1e300;
+        double best = 1e300, best_lds = 1e300;
         int bt = e->tile[bk], bs = e->split[bk];
         for (int T : nbh::kTiles) {
             for (int S : nbh::kSplits) {
@@ -974,6 +979,11 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
                     best = ms;
                     bt = T;
                     bs = S;
+                }
+                if (T > 0 && ms < best_lds) {
+                    best_lds = ms;
+                    lds_tile[bk] = T;
+                    lds_split[bk] = S;
                 }
             }
             if (rc) break;
@@ -996,6 +1006,25 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
             }
         }
         e->ipl = bq;
+    }
+    // the buckets were timed alone; together, the scalar-load kernel runs as a
+    // launch of its own beside the persistent one instead of sharing its
+    // tail: keep it only if the whole list is faster with it
+    bool any_scalar = false;
+    for (int b = 0; b < NBH_ENS_BUCKETS; ++b) any_scalar |= e->tile[b] == 0;
+    if (rc == NBH_OK && any_scalar) {
+        double with = 0.0, without = 0.0;
+        int keep_t[NBH_ENS_BUCKETS], keep_s[NBH_ENS_BUCKETS];
+        std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, keep_t);
+        std::copy(e->split, e->split + NBH_ENS_BUCKETS, keep_s);
+        if (!(rc = rebuild(e)) && !(rc = time_list(&with))) {
+            for (int b = 0; b < NBH_ENS_BUCKETS; ++b)
+                if (e->tile[b] == 0) e->tile[b] = lds_tile[b], e->split[b] = lds_split[b];
+            if (!(rc = rebuild(e)) && !(rc = time_list(&without)) && with < without) {
+                std::copy(keep_t, keep_t + NBH_ENS_BUCKETS, e->tile);
+                std::copy(keep_s, keep_s + NBH_ENS_BUCKETS, e->split);
+            }
+        }
     }
     c->profiling = prof;
     (void)hipEventDestroy(a);
