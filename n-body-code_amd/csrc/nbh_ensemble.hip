@@ -456,6 +456,12 @@ struct nbh_ensemble {
     bool uploaded = false, dt_set = false;
     hipGraphExec_t gexec = nullptr;
     int g_steps = 0;
+    // second stream (default priority) on which the persistent launch runs
+    // beside the scalar-load launch, forked from and joined to the
+    // context's stream with events (captured into the step graph as two
+    // branches)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -655,12 +661,30 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
         if (rc) return rc;
         NBH_HIP(hipEventRecord(ev.ep.a, c->stream));
     }
-    nbh::k_zero_u32<<<1, 64, 0, c->stream>>>(a.counter);
-    NBH_HIP(hipGetLastError());
     const bool mixed = c->cfg.precision == NBH_MIXED;
     const bool mask = e->any_mask[mixed];
     if (mixed) {
         nbh::k_ens_vref<<<e->nsys, nbh::kLanes, 0, c->stream>>>(a.s, a.sys, e->nsys, e->vref.d());
+        NBH_HIP(hipGetLastError());
+    }
+    // with both launches present, the persistent one runs on the side
+    // stream, so that its items fill the scalar-load launch's tail
+    const bool fork = e->nbig > 0 && e->nitems > 0;
+    hipStream_t ps = c->stream;
+    if (fork) {
+        NBH_HIP(hipEventRecord(e->ev_fork, c->stream));
+        NBH_HIP(hipStreamWaitEvent(e->side, e->ev_fork, 0));
+        ps = e->side;
+    }
+    nbh::k_zero_u32<<<1, 64, 0, ps>>>(a.counter);
+    NBH_HIP(hipGetLastError());
+    const void* k = nbh::pick_ens_kernel(mixed, e->maxtile, mask, e->list_ipl);
+    if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d", e->maxtile);
+    void* args[] = {&a};
+    NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, ps));
+    if (e->nreduce > 0) {
+        nbh::k_ens_reduce<<<e->nreduce, nbh::kLanes, 0, ps>>>(static_cast<const EnsReduce*>(e->reduces.p),
+                                                              e->nreduce, e->partial.d(), out);
         NBH_HIP(hipGetLastError());
     }
     if (e->nbig > 0) {  // the large systems' source records, then their scalar-load kernel
@@ -682,14 +706,9 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
             nbh::k_ens_big<false, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
         NBH_HIP(hipGetLastError());
     }
-    const void* k = nbh::pick_ens_kernel(mixed, e->maxtile, mask, e->list_ipl);
-    if (!k) return fail(NBH_ERR_ARG, "no ensemble kernel for tile %d", e->maxtile);
-    void* args[] = {&a};
-    NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, c->stream));
-    if (e->nreduce > 0) {
-        nbh::k_ens_reduce<<<e->nreduce, nbh::kLanes, 0, c->stream>>>(static_cast<const EnsReduce*>(e->reduces.p),
-                                                                      e->nreduce, e->partial.d(), out);
-        NBH_HIP(hipGetLastError());
+    if (fork) {
+        NBH_HIP(hipEventRecord(e->ev_join, e->side));
+        NBH_HIP(hipStreamWaitEvent(c->stream, e->ev_join, 0));
     }
     if (e->nsym > 0)
         NBH_HIP(nbh::launch_ens_sym(c->cfg.precision, a.s, out, static_cast<const nbh::SymItem*>(e->symitems.p),
@@ -741,6 +760,15 @@ int nbh_ens_create(nbh_ctx* ctx, int nsys, const int* counts, const double* eps2
     if (!e) return fail(NBH_ERR_ALLOC, "ensemble allocation failed");
     e->ctx = ctx;
     e->sym = ctx->cfg.pairing == NBH_PAIR_SYMMETRIC;
+    {
+        hipError_t he = hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming);
+        if (he != hipSuccess) {
+            nbh_ens_destroy(e);
+            return fail(NBH_ERR_HIP, "ensemble stream/event creation failed: %s", hipGetErrorString(he));
+        }
+    }
     e->nsys = nsys;
     e->total = (int)total;
     e->cnt.assign(counts, counts + nsys);
@@ -782,7 +810,11 @@ int nbh_ens_destroy(nbh_ensemble* e)
     if (!e) return NBH_OK;
     (void)hipSetDevice(e->ctx->cfg.device);
     (void)hipStreamSynchronize(e->ctx->stream);
+    if (e->side) (void)hipStreamSynchronize(e->side);
     drop_graph(e);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+    if (e->side) (void)hipStreamDestroy(e->side);
     for (DevBuf* b : {&e->sys, &e->state, &e->f0, &e->f1, &e->items, &e->reduces, &e->partial, &e->counter,
                       &e->dt, &e->ekin, &e->epot, &e->symitems, &e->vref, &e->bigitems, &e->biglist, &e->rec})
         b->release();
@@ -1007,24 +1039,33 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
         }
         e->ipl = bq;
     }
-    // the buckets were timed alone; together, the scalar-load kernel runs as a
-    // launch of its own beside the persistent one instead of sharing its
-    // tail: keep it only if the whole list is faster with it
-    bool any_scalar = false;
-    for (int b = 0; b < NBH_ENS_BUCKETS; ++b) any_scalar |= e->tile[b] == 0;
-    if (rc == NBH_OK && any_scalar) {
-        double with = 0.0, without = 0.0;
-        int keep_t[NBH_ENS_BUCKETS], keep_s[NBH_ENS_BUCKETS];
-        std::copy(e->tile, e->tile + NBH_ENS_BUCKETS, keep_t);
-        std::copy(e->split, e->split + NBH_ENS_BUCKETS, keep_s);
-        if (!(rc = rebuild(e)) && !(rc = time_list(&with))) {
-            for (int b = 0; b < NBH_ENS_BUCKETS; ++b)
-                if (e->tile[b] == 0) e->tile[b] = lds_tile[b], e->split[b] = lds_split[b];
-            if (!(rc = rebuild(e)) && !(rc = time_list(&without)) && with < without) {
-                std::copy(keep_t, keep_t + NBH_ENS_BUCKETS, e->tile);
-                std::copy(keep_s, keep_s + NBH_ENS_BUCKETS, e->split);
+    // the buckets were timed alone; together, the scalar-load kernel is a
+    // launch of its own beside the persistent one (on a second stream), so
+    // the whole list is timed three ways -- as tuned, with every tile-0
+    // bucket back on its best LDS tile, and with only the largest tile-0
+    // bucket kept on the scalar-load kernel -- and the fastest is kept
+    int last0 = -1;
+    for (int b = 0; b < NBH_ENS_BUCKETS; ++b)
+        if (e->tile[b] == 0) last0 = b;
+    if (rc == NBH_OK && last0 >= 0) {
+        int cand_t[3][NBH_ENS_BUCKETS], cand_s[3][NBH_ENS_BUCKETS];
+        for (int v = 0; v < 3; ++v)
+            for (int b = 0; b < NBH_ENS_BUCKETS; ++b) {
+                const bool revert = e->tile[b] == 0 && (v == 1 || (v == 2 && b != last0));
+                cand_t[v][b] = revert ? lds_tile[b] : e->tile[b];
+                cand_s[v][b] = revert ? lds_split[b] : e->split[b];
             }
+        double best = 1e300;
+        int bv = 0;
+        for (int v = 0; v < 3 && rc == NBH_OK; ++v) {
+            std::copy(cand_t[v], cand_t[v] + NBH_ENS_BUCKETS, e->tile);
+            std::copy(cand_s[v], cand_s[v] + NBH_ENS_BUCKETS, e->split);
+            double ms = 0.0;
+            if ((rc = rebuild(e)) || (rc = time_list(&ms))) break;
+            if (ms < best) best = ms, bv = v;
         }
+        std::copy(cand_t[bv], cand_t[bv] + NBH_ENS_BUCKETS, e->tile);
+        std::copy(cand_s[bv], cand_s[bv] + NBH_ENS_BUCKETS, e->split);
     }
     c->profiling = prof;
     (void)hipEventDestroy(a);
