@@ -350,3 +350,22 @@ def test_symmetric_context_with_scalar_load_buckets(nb, oracle, symset, precisio
         e.upload(state)
         _, f = e.download()
         check_systems(e, f, state, oracle, eps2, mixed=(precision == "mixed"))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_two_stream_step_graph_matches_eager(nb, small, precision):
+    """The largest bucket on the scalar-load kernel and the others on the
+    persistent kernel: the two launches run on two streams, forked and joined
+    with events, and a captured step graph holds both branches.  Sixteen
+    steps replayed from the graph equal sixteen eager steps bit for bit."""
+    counts, eps2, state = small
+    out = {}
+    for graph in (False, True):
+        with nb.Backend(precision) as b, nb.Ensemble(b, counts, eps2) as e:
+            e.set_tuning([64, 128, 256, 0], [1, 2, 1, 1])
+            e.upload(state)
+            e.set_dt(1.0 / 256)
+            e.step(16, graph=graph)
+            out[graph] = e.download()
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][1], out[False][1])
