@@ -276,14 +276,9 @@ int fit_scratch(const nbh_ctx* c, int n, nbh::Shape& sh)
 {
     const int prec = c->cfg.precision;
     if (scratch_need(prec, n, sh) <= nbh::kScratchBytes) return NBH_OK;
-    if (c->cfg.jsplit > 0 || c->cfg.sources != NBH_SRC_AUTO)
-        return fail(NBH_ERR_ARG,
-                    "n=%d with the configured j-split %d and %s sources needs %zu bytes of scratch; "
-                    "nbh_force_device_noalloc has %zu (leave jsplit and sources automatic, or use "
-                    "nbh_force_device)",
-                    n, sh.jsplit, sh.scalar ? "scalar-load" : "LDS-tile", scratch_need(prec, n, sh), nbh::kScratchBytes);
-    for (int scalar = sh.scalar; scalar >= 0; --scalar) {
-        for (int split = sh.jsplit; split >= 1; split = split > 1 ? split / 2 : 0) {
+    const bool keep_split = c->cfg.jsplit > 0, keep_src = c->cfg.sources != NBH_SRC_AUTO;
+    for (int scalar = sh.scalar; scalar >= (keep_src ? sh.scalar : 0); --scalar) {
+        for (int split = sh.jsplit; split >= (keep_split ? sh.jsplit : 1); split = split > 1 ? split / 2 : 0) {
             const nbh::Shape t = make_shape(n, sh.block, sh.ipl, split, scalar != 0);
             if (scratch_need(prec, n, t) <= nbh::kScratchBytes) {
                 sh = t;
@@ -291,8 +286,11 @@ int fit_scratch(const nbh_ctx* c, int n, nbh::Shape& sh)
             }
         }
     }
-    return fail(NBH_ERR_ARG, "n=%d does not fit the %zu-byte scratch of nbh_force_device_noalloc", n,
-                nbh::kScratchBytes);
+    return fail(NBH_ERR_ARG,
+                "n=%d with the configured %s needs %zu bytes of scratch; nbh_force_device_noalloc has %zu "
+                "(leave jsplit and sources automatic, or use nbh_force_device)",
+                n, keep_split && keep_src ? "j-split and source path" : (keep_split ? "j-split" : "source path"),
+                scratch_need(prec, n, sh), nbh::kScratchBytes);
 }
 
 }  // namespace

@@ -135,6 +135,19 @@ def test_noalloc_narrows_large_automatic_shapes(nb, hip, oracle, precision):
                 assert after == before, (before, after)
 
 
+def test_noalloc_keeps_a_forced_source_path_and_narrows_the_split(nb, hip):
+    """LDS sources forced, j-split automatic, n = 65536: the call keeps the
+    LDS path and runs fewer splits; FP64 sums equal nbh_force_device's to
+    rounding of the split order."""
+    n = 65536
+    with nb.Backend("fp64", 1e-4, sources="lds") as b:
+        s = nb.plummer(n, 8)
+        ref, _, _ = evaluate(nb, hip, b, s, "nbh_force_device")
+        got, _, _ = evaluate(nb, hip, b, s, "nbh_force_device_noalloc")
+    assert body_relerr(got[:3], ref[:3]).max() <= 1e-12
+    assert body_relerr(got[3:], ref[3:]).max() <= 1e-12
+
+
 def test_noalloc_rejects_forced_shape_beyond_scratch(nb, hip):
     """A forced j-split whose partial sums exceed the 64 MiB scratch is
     rejected with an argument error before anything runs, and the outputs
