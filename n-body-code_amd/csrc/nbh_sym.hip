@@ -129,43 +129,78 @@ struct SymMixed {
             I.vz[k] = (float)(s.vz[i] - vr.z);
         }
     }
+    // One source against the K targets, written one operation at a time
+    // across the targets so that their K dependency chains (FP64 difference,
+    // convert, |d|^2, rsqrt, r^3, ...) reach the scheduler interleaved.  With
+    // the chains written target after target, the compiler, held to 96 VGPRs
+    // for five waves, ran the first target's chain nearly serially (one FP64
+    // temporary, every instruction waiting on the last): the interleaved
+    // form is 6% faster at n = 2^20 with the same 96 VGPRs.
     template <int R>
     static __device__ __forceinline__ void step(const Rec* __restrict__ row, const Targets& I, Eps eps2,
                                                 Sum (&ai)[K][6], Sum (&aj)[6])
     {
         const Rec b = row[R];
+        float dx[K], dy[K], dz[K], ux[K], uy[K], uz[K], sq[K], vr[K], r[K], r2[K], r3[K], al[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) dx[k] = (float)(b.x - I.x[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) dy[k] = (float)(b.y - I.y[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) dz[k] = (float)(b.z - I.z[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) sq[k] = fmaf(dz[k], dz[k], eps2);
+#pragma unroll
+        for (int k = 0; k < K; ++k) sq[k] = fmaf(dy[k], dy[k], sq[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) sq[k] = fmaf(dx[k], dx[k], sq[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) r[k] = __builtin_amdgcn_rsqf(sq[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ux[k] = b.vx - I.vx[k];
+            uy[k] = b.vy - I.vy[k];
+            uz[k] = b.vz - I.vz[k];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) vr[k] = dz[k] * uz[k];
+#pragma unroll
+        for (int k = 0; k < K; ++k) vr[k] = fmaf(dy[k], uy[k], vr[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) vr[k] = fmaf(dx[k], ux[k], vr[k]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) r2[k] = r[k] * r[k];
+#pragma unroll
+        for (int k = 0; k < K; ++k) r3[k] = r[k] * r2[k];
+#pragma unroll
+        for (int k = 0; k < K; ++k) al[k] = (-3.0f * r2[k]) * vr[k];
+        // u becomes w = u - 3 (d.u) r^2 d in place
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ux[k] = fmaf(al[k], dx[k], ux[k]);
+            uy[k] = fmaf(al[k], dy[k], uy[k]);
+            uz[k] = fmaf(al[k], dz[k], uz[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float mj3 = b.m * r3[k];
+            ai[k][0] = fmaf(mj3, dx[k], ai[k][0]);
+            ai[k][1] = fmaf(mj3, dy[k], ai[k][1]);
+            ai[k][2] = fmaf(mj3, dz[k], ai[k][2]);
+            ai[k][3] = fmaf(mj3, ux[k], ai[k][3]);
+            ai[k][4] = fmaf(mj3, uy[k], ai[k][4]);
+            ai[k][5] = fmaf(mj3, uz[k], ai[k][5]);
+        }
         float cs[6];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const float dx = (float)(b.x - I.x[k]);
-            const float dy = (float)(b.y - I.y[k]);
-            const float dz = (float)(b.z - I.z[k]);
-            const float ux = b.vx - I.vx[k];
-            const float uy = b.vy - I.vy[k];
-            const float uz = b.vz - I.vz[k];
-            const float s = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-            const float vr = fmaf(dx, ux, fmaf(dy, uy, dz * uz));
-            const float r = __builtin_amdgcn_rsqf(s);
-            const float r2 = r * r;
-            const float r3 = r * r2;
-            const float mj3 = b.m * r3;
-            const float mi3 = I.m[k] * r3;
-            const float al = (-3.0f * r2) * vr;
-            const float wx = fmaf(al, dx, ux);
-            const float wy = fmaf(al, dy, uy);
-            const float wz = fmaf(al, dz, uz);
-            ai[k][0] = fmaf(mj3, dx, ai[k][0]);
-            ai[k][1] = fmaf(mj3, dy, ai[k][1]);
-            ai[k][2] = fmaf(mj3, dz, ai[k][2]);
-            ai[k][3] = fmaf(mj3, wx, ai[k][3]);
-            ai[k][4] = fmaf(mj3, wy, ai[k][4]);
-            ai[k][5] = fmaf(mj3, wz, ai[k][5]);
+            const float mi3 = I.m[k] * r3[k];
             if (k == 0) {
-                cs[0] = mi3 * dx; cs[1] = mi3 * dy; cs[2] = mi3 * dz;
-                cs[3] = mi3 * wx; cs[4] = mi3 * wy; cs[5] = mi3 * wz;
+                cs[0] = mi3 * dx[k]; cs[1] = mi3 * dy[k]; cs[2] = mi3 * dz[k];
+                cs[3] = mi3 * ux[k]; cs[4] = mi3 * uy[k]; cs[5] = mi3 * uz[k];
             } else {
-                cs[0] = fmaf(mi3, dx, cs[0]); cs[1] = fmaf(mi3, dy, cs[1]); cs[2] = fmaf(mi3, dz, cs[2]);
-                cs[3] = fmaf(mi3, wx, cs[3]); cs[4] = fmaf(mi3, wy, cs[4]); cs[5] = fmaf(mi3, wz, cs[5]);
+                cs[0] = fmaf(mi3, dx[k], cs[0]); cs[1] = fmaf(mi3, dy[k], cs[1]); cs[2] = fmaf(mi3, dz[k], cs[2]);
+                cs[3] = fmaf(mi3, ux[k], cs[3]); cs[4] = fmaf(mi3, uy[k], cs[4]); cs[5] = fmaf(mi3, uz[k], cs[5]);
             }
         }
 #pragma unroll
