@@ -13,7 +13,7 @@ correct), run by the device-resident integrator.  Four legs run:
   mixed          -- FP64 position differences rounded to FP32 (velocities
                     taken relative to the centre-of-mass velocity in FP64,
                     then rounded), FP32 r^2 / rsqrt / jerk, symmetric
-                    pairing: FP32 sums over runs of 128 sources (targets) or
+                    pairing: FP32 sums over runs of 256 sources (targets) or
                     one tile of 64 sources x 4 waves (sources), carried on in
                     FP64 with FP64 atomic adds: the headline `value`;
   mixed_directed -- the mixed arithmetic on directed pairs, FP32 sums over

@@ -37,7 +37,7 @@
 //     outputs with FP64 atomics;
 //   * target sums run over the sources of FL consecutive tiles and are then
 //     added to the outputs with FP64 atomics (the single-system kernel, FL =
-//     2 mixed, 8 FP64: no totals held in registers, which leaves room for
+//     4 mixed, 8 FP64: no totals held in registers, which leaves room for
 //     three targets per lane at five waves per SIMD in mixed), or over one tile and then into
 //     per-lane totals that go out once the workgroup's block pairs are done
 //     (the ensemble's kernel, FL = 0; mixed: FP32 sums folded into an FP32
@@ -95,11 +95,13 @@ template <int K>
 struct SymMixed {
     static constexpr int kWaves = 5;
     // tiles per flush of the target sums in the single-system kernel: FP32
-    // runs of 128 sources.  Measured at n = 2^20 (three targets per lane,
-    // five waves): one tile per flush ran no faster than two targets per lane
-    // with two-sum totals in registers, two tiles 3.5% faster, four 5% faster
-    // but with twice the largest acceleration error
-    static constexpr int kFlushTiles = 2;
+    // runs of 256 sources between FP64 atomic adds.  Measured at n = 2^20
+    // (three targets per lane, five waves, eps2 = 1e-6) against two tiles:
+    // 2.2% faster (three tiles 1.4%), the largest acceleration error against
+    // FP64 7.2e-7 instead of 6.3e-7 (n = 65536: 1.3e-6 instead of 9.0e-7),
+    // the median 2.25e-8 instead of 2.23e-8.  One tile ran no faster than two
+    // targets per lane with two-sum totals in registers.
+    static constexpr int kFlushTiles = 4;
     using Rec = JBMX;
     using Sum = float;
     using Eps = float;
