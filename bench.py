@@ -70,7 +70,7 @@ DT = 1.0 / 2048
 FLOP_PER_PAIR = 60
 FLOP_PER_SYM_PAIR = 73
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector
-SYM_FLUSH_TILES = {"fp64": 8, "mixed": 2}  # tiles per flush of the symmetric kernels' target sums
+SYM_FLUSH_TILES = {"fp64": 8, "mixed": 4}  # tiles per flush of the symmetric kernels' target sums
 
 
 def body_relerr(a, b):
