@@ -321,6 +321,10 @@ __global__ __launch_bounds__(kBigBlock) void k_ens_big(State s, Forces out, cons
 {
     const EnsBig it = items[blockIdx.x];
     const int t = threadIdx.x;
+    // a system's last workgroup holds fewer than 256 targets: waves with
+    // none of them leave at once (wave-uniform, and the kernel has no
+    // barrier), so only the partial wave's idle lanes repeat a target
+    if ((t & ~63) >= it.icount) return;
     const int i = it.i0 + (t < it.icount ? t : 0);
     const double eps2 = sys[it.sys].eps2;
     const double xi = s.x[i], yi = s.y[i], zi = s.z[i];
