@@ -331,7 +331,9 @@ __global__ __launch_bounds__(BLOCK) void k_force_f64(PairArgs a, const JB64* __r
 template <int BLOCK, int IPL, bool MASK>
 __global__ __launch_bounds__(BLOCK) void k_force_mixed(PairArgs a, const JBMX* __restrict__ rec)
 {
-    constexpr int UNROLL = IPL == 1 ? 8 : 4;
+    // one target per lane: 16 sources per unrolled body (54 VGPRs, as with
+    // 8, and 0.7% faster at n = 2^20)
+    constexpr int UNROLL = IPL == 1 ? 16 : 4;
     const int t = threadIdx.x;
     const int base = blockIdx.x * BLOCK * IPL + t;
     const float eps2 = (float)a.eps2;
