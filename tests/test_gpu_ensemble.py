@@ -321,8 +321,11 @@ def test_scalar_load_kernel_bitwise_equals_lds_tiles(nb, oracle, precision):
     of one system per four-wave workgroup, sources read with scalar loads):
     the same sums, bit for bit, as the persistent LDS-tile kernel with one
     split, for the large bucket alone and for every bucket; and against the
-    oracle."""
-    counts = np.array([1, 2, 63, 64, 65, 300, 1025, 1100, 2048, 2049, 3001, 4096], dtype=np.int32)
+    oracle.  A system's last workgroup then holds 1..256 targets, its waves
+    past the last one leaving at once: 128/129, 192/193, 255 and 257 put that
+    edge on every wave boundary."""
+    counts = np.array([1, 2, 63, 64, 65, 128, 129, 192, 193, 255, 257, 300, 1025, 1100, 2048, 2049, 3001, 4096],
+                      dtype=np.int32)
     eps2 = (1.0 / (4.0 * counts.astype(float) ** (1 / 3))) ** 2
     state = nb.ensemble_state(counts, 29)
     out = {}
