@@ -2,8 +2,8 @@
 // apply it to both bodies (Newton's third law), as the reference's CPU loops
 // do with their j < i triangle (src/kernel.cpp:1126 for _F32, :1187 for
 // FP64).  Two precisions share the schedule: mixed (FP64 position
-// differences rounded to FP32, FP32 from there on, two-sum target totals) and
-// FP64 (the directed FP64 kernel's arithmetic).
+// differences rounded to FP32, FP32 from there on) and FP64 (the directed
+// FP64 kernel's arithmetic).
 //
 // Layout of the work:
 //   * bodies are cut into nr = n / B full blocks of B = 64*K*W (K targets per
@@ -90,7 +90,9 @@ __device__ __forceinline__ double row_rot(double v)
 }
 
 // Mixed precision: FP64 position differences rounded to FP32, FP32 from
-// there on; per-tile FP32 target sums folded into a two-sum (hi, lo) pair.
+// there on.  Target sums: FP32 runs of kFlushTiles tiles added to the
+// outputs in FP64 (one system), or per-tile FP32 sums folded into a two-sum
+// (hi, lo) pair (the ensemble's kernel).
 template <int K>
 struct SymMixed {
     static constexpr int kWaves = 5;
