@@ -66,6 +66,30 @@ __device__ __forceinline__ double pad_x(int j)
     return -1e18 * (1.0 + 1e-3 * (j & 1023));
 }
 
+// FP64 atomic add to element idx of one output array.  The base is a kernel
+// argument (wave-uniform) and the offset a 32-bit byte count, so the address
+// is an SGPR pair plus one VGPR: no 64-bit address registers are kept across
+// the step loop (they were spilled, and each reload waited for every atomic
+// in flight).  Needs n <= 2^29 (sym_supported).
+__device__ __forceinline__ void out_add(double* base, unsigned idx, double v)
+{
+    unsigned off = idx * 8u;
+    // opaque to the optimiser: the address is formed here, at the add, and
+    // not hoisted out of the loops into 64-bit registers
+    asm volatile("" : "+v"(off));
+    unsafeAtomicAdd(reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off), v);
+}
+
+// Lane index within the wave, recomputed where it is used (two VALU
+// instructions): the compiler cannot fold it into a value computed before
+// the step loop and keep that value live, or spilled, across the loop.
+__device__ __forceinline__ int lane_now()
+{
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
 // Row rotation by R lanes: lane k of each 16-lane row receives the value of
 // lane (k - R) mod 16 of the same row.
 template <int R>
@@ -374,7 +398,7 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
 
     const int t = threadIdx.x;
     const int lane = t & 63;
-    const int w = t >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: an SGPR
     const int p = lane & 15;
     const typename T::Eps eps2 = (typename T::Eps)eps2d;
     const int ibase = P * B + w * 64 * K + lane;
@@ -423,6 +447,18 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
                 part[buf][w][g * 16 + p][c] = v;
             }
         }
+        // the next tile is loaded after the arithmetic, not held in registers
+        // across it (12 VGPRs); the other resident workgroups cover the load.
+        // It comes before the target atomics below, since a wait for a load
+        // also waits for every memory operation issued before it
+        // (lane indices past the step loop come from lane_now(), so that none
+        // is held in a register, or spilled, across it)
+        if (more && w == 0) {
+            const int l = lane_now();
+            st = T::load_rec(s, tile_src(tt + 1) + l, n, vr);
+            dup[buf ^ 1][l >> 4][l & 15] = st;
+            dup[buf ^ 1][l >> 4][(l & 15) + 16] = st;
+        }
         if constexpr (FL == 0) {
             T::fold(tot, ai);
 #pragma unroll
@@ -434,28 +470,22 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
             for (int k = 0; k < K; ++k)
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
-                    unsafeAtomicAdd(&out.c[c][ibase + k * 64], (double)ai[k][c]);
+                    out_add(out.c[c], (unsigned)(P * B + w * 64 * K + lane_now() + k * 64), (double)ai[k][c]);
                     ai[k][c] = Sum(0);
                 }
         }
-        // the next tile is loaded after the arithmetic, not held in registers
-        // across it (12 VGPRs); the other resident workgroups cover the load
-        if (more && t < kSymTile) {
-            st = T::load_rec(s, tile_src(tt + 1) + t, n, vr);
-            dup[buf ^ 1][t >> 4][t & 15] = st;
-            dup[buf ^ 1][t >> 4][(t & 15) + 16] = st;
-        }
         __syncthreads();
-        // the tile's source totals: waves combined in wave order
+        // the tile's source totals, waves combined in wave order: wave w adds
+        // the components c = w (mod W) of the tile's 64 sources, lane j source j
         const int j0 = tile_src(tt);
-        for (int e = t; e < kSymTile * 6; e += 64 * W) {
-            const int c = e / kSymTile;
-            const int j = e % kSymTile;
-            if (j0 + j < n) {
+        const int l = lane_now();
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (c % W == w && j0 + l < n) {
                 double sum = 0.0;
 #pragma unroll
-                for (int v = 0; v < W; ++v) sum += (double)part[buf][v][j][c];
-                unsafeAtomicAdd(&out.c[c][j0 + j], sum);
+                for (int v = 0; v < W; ++v) sum += (double)part[buf][v][l][c];
+                out_add(out.c[c], (unsigned)(j0 + l), sum);
             }
         }
     }
@@ -463,7 +493,7 @@ __device__ __forceinline__ void sym_block_pairs(const State& s, const Forces& ou
 #pragma unroll
         for (int k = 0; k < K; ++k)
 #pragma unroll
-            for (int c = 0; c < 6; ++c) unsafeAtomicAdd(&out.c[c][ibase + k * 64], T::total(tot, k, c));
+            for (int c = 0; c < 6; ++c) out_add(out.c[c], (unsigned)(ibase + k * 64), T::total(tot, k, c));
     }
 }
 
@@ -527,7 +557,8 @@ int sym_block(int precision) { return precision == 0 ? 64 * kF64K * kF64W : 64 *
 
 int sym_targets_per_lane(int precision) { return precision == 0 ? kF64K : kMixK; }
 
-bool sym_supported(int precision, int n) { return n >= 2 * sym_block(precision); }
+// n <= 2^29: the kernels address the outputs with 32-bit byte offsets
+bool sym_supported(int precision, int n) { return n >= 2 * sym_block(precision) && n <= (1 << 29); }
 
 int ens_sym_block(int precision) { return precision == 0 ? 64 * kF64EnsK * kF64EnsW : 64 * kMixEnsK * kMixEnsW; }
 
