@@ -1,8 +1,10 @@
 // nbh_ensemble.hip -- many independent systems advanced together.
 //
 // Layout: the systems' SoA arrays are concatenated in system order (system k
-// occupies [off_k, off_k + n_k)), so the predictor and corrector kernels of
-// the single-system path run unchanged over all bodies at once.
+// occupies [off_k, off_k + n_k)).  A step is the force evaluation at the
+// predicted state, then one fused kernel per system (k_ens_advance): the
+// corrector, f0 <- f1, the next prediction, the reference velocity and the
+// source records, with the single-system kernels' arithmetic.
 //
 // Force evaluation: one persistent launch of one-wave (64-lane) workgroups.
 // Each wave takes work items from a device queue with an atomic counter:
@@ -15,6 +17,7 @@
 // once the counter passes the item count, so the grid always drains.
 #include "nbh_ctx.h"
 #include "nbh_pair.h"
+#include "nbh_hermite.h"
 
 #include <algorithm>
 #include <chrono>
@@ -305,6 +308,109 @@ __global__ __launch_bounds__(kBigBlock) void k_ens_pack(State s, const EnsSys* _
     }
 }
 
+// The ensemble's step after its force evaluation.  The prediction of a step
+// is double-buffered (pred[0], pred[1], and likewise the records of the
+// scalar-load kernel's systems): step n reads buffer p and writes the next
+// prediction into 1 - p, so a kernel can write the next prediction of its
+// own targets while other workgroups of the same system still read this
+// step's sources.
+//   * Systems on the scalar-load kernel (k_ens_big with ADVANCE): each
+//     workgroup, once its targets' forces are summed, runs the corrector,
+//     f0 <- f1 and the next prediction for those targets, and packs their
+//     next records: no pass over memory of its own.
+//   * Every other system: k_ens_advance, one thread per body.
+//   * The first step of a run: k_ens_prologue predicts from the current
+//     state and f0, and packs the records.
+// The arithmetic is the single-system kernels' (nbh_hermite.h), so the
+// states are bitwise those of k_predict and k_correct.  The reference
+// velocity of each system (mixed precision) is the centre-of-mass velocity
+// of the state the last force evaluation with preparation was made on
+// (upload, autotune): pairwise forces conserve it, so it stays the
+// centre-of-mass velocity to rounding, without a reduction per step.
+struct EnsNext {
+    double* cp[3];  // corrected positions (the current state)
+    double* cv[3];  // corrected velocities
+    double* np[3];  // next predicted positions
+    double* nv[3];  // next predicted velocities
+    const double* m;
+    Forces f0;      // forces at the start of the step, replaced by this step's
+    const double* dt;
+    void* rec;      // next records (scalar-load systems), indexed like the bodies
+};
+
+// Corrector of one body from its prediction (p, u) and the forces f1 at it,
+// f0 <- f1, then the next prediction; returns the corrected state in (p, u)
+// replaced by the next prediction.
+__device__ __forceinline__ void advance_body(const EnsNext& nx, int i, double dt, double invdt2, double invdt3,
+                                             const double (&f1)[6], double (&p)[3], double (&u)[3])
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double a2, a3;
+        hermite_correct_axis(dt, invdt2, invdt3, nx.f0.c[c][i], nx.f0.c[3 + c][i], f1[c], f1[3 + c], p[c], u[c],
+                             a2, a3);
+        nx.cp[c][i] = p[c];
+        nx.cv[c][i] = u[c];
+        nx.f0.c[c][i] = f1[c];
+        nx.f0.c[3 + c][i] = f1[3 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        hermite_predict_axis(dt, f1[c], f1[3 + c], p[c], u[c]);
+        nx.np[c][i] = p[c];
+        nx.nv[c][i] = u[c];
+    }
+}
+
+constexpr int kAdvBlock = 256;
+
+// One thread per body of the systems not on the scalar-load kernel
+// (bodysys[i] < 0): the prediction (pp, pv) and the forces f1 at it.
+__global__ __launch_bounds__(kAdvBlock) void k_ens_advance(EnsNext nx, State pred, Forces f1,
+                                                         const int* __restrict__ bodysys, int total)
+{
+    const int i = blockIdx.x * kAdvBlock + threadIdx.x;
+    if (i >= total || bodysys[i] >= 0) return;
+    const double dt = *nx.dt;
+    const double invdt2 = 1.0 / (dt * dt);
+    const double invdt3 = 1.0 / (dt * dt * dt);
+    double f[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) f[c] = f1.c[c][i];
+    double p[3] = {pred.x[i], pred.y[i], pred.z[i]};
+    double u[3] = {pred.vx[i], pred.vy[i], pred.vz[i]};
+    advance_body(nx, i, dt, invdt2, invdt3, f, p, u);
+}
+
+// First prediction of a run, one thread per body: from the current state
+// (nx.cp, nx.cv) and f0 into nx.np, nx.nv, and the records of the bodies
+// whose system (bodysys[i] >= 0) is on the scalar-load kernel.
+template <bool MIXED>
+__global__ __launch_bounds__(kAdvBlock) void k_ens_prologue(EnsNext nx, const int* __restrict__ bodysys,
+                                                          const double* __restrict__ vref, int total)
+{
+    const int i = blockIdx.x * kAdvBlock + threadIdx.x;
+    if (i >= total) return;
+    const double dt = *nx.dt;
+    double p[3], u[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        p[c] = nx.cp[c][i];
+        u[c] = nx.cv[c][i];
+        hermite_predict_axis(dt, nx.f0.c[c][i], nx.f0.c[3 + c][i], p[c], u[c]);
+        nx.np[c][i] = p[c];
+        nx.nv[c][i] = u[c];
+    }
+    const int k = bodysys[i];
+    if (k >= 0 && nx.rec) {
+        if constexpr (MIXED)
+            static_cast<JBMX*>(nx.rec)[i] = make_jbmx(p[0], p[1], p[2], nx.m[i], u[0], u[1], u[2],
+                                                      load_vref(vref + 3 * k));
+        else
+            static_cast<JB64*>(nx.rec)[i] = make_jb64(p[0], p[1], p[2], nx.m[i], u[0], u[1], u[2]);
+    }
+}
+
 // Scalar-load kernel for the large systems: the single-system directed
 // kernel's structure (one target per lane, four waves per workgroup, the
 // source record at a wave-uniform index so that it is read with scalar
@@ -313,11 +419,13 @@ __global__ __launch_bounds__(kBigBlock) void k_ens_pack(State s, const EnsSys* _
 // scalar cache then serves to all four.  Each target sums its sources in
 // order from j0, FP32 runs of kFold folded into the two-sum (hi, lo) totals
 // in mixed precision: the same sums, bit for bit, as the persistent kernel
-// over the same source range.
-template <bool MIXED, bool MASK>
+// over the same source range.  ADVANCE (a step): instead of writing the
+// forces to `out`, each target is corrected and predicted for the next step
+// (advance_body) and its next record packed, into nx.
+template <bool MIXED, bool MASK, bool ADVANCE>
 __global__ __launch_bounds__(kBigBlock) void k_ens_big(State s, Forces out, const EnsSys* __restrict__ sys,
                                                      const EnsBig* __restrict__ items, const void* __restrict__ rec,
-                                                     const double* __restrict__ vref)
+                                                     const double* __restrict__ vref, EnsNext nx)
 {
     const EnsBig it = items[blockIdx.x];
     const int t = threadIdx.x;
@@ -329,8 +437,9 @@ __global__ __launch_bounds__(kBigBlock) void k_ens_big(State s, Forces out, cons
     const double eps2 = sys[it.sys].eps2;
     const double xi = s.x[i], yi = s.y[i], zi = s.z[i];
     double f[6];
+    VRef vr{0.0, 0.0, 0.0};
     if constexpr (MIXED) {
-        const VRef vr = load_vref(vref + 3 * it.sys);
+        vr = load_vref(vref + 3 * it.sys);
         const float vxi = (float)(s.vx[i] - vr.x), vyi = (float)(s.vy[i] - vr.y), vzi = (float)(s.vz[i] - vr.z);
         const float e2 = (float)eps2;
         const JBMX* __restrict__ r = static_cast<const JBMX*>(rec);
@@ -362,8 +471,19 @@ __global__ __launch_bounds__(kBigBlock) void k_ens_big(State s, Forces out, cons
         for (int c = 0; c < 6; ++c) f[c] = acc[c];
     }
     if (t < it.icount) {
+        if constexpr (ADVANCE) {
+            const double dt = *nx.dt;
+            double p[3] = {xi, yi, zi};
+            double u[3] = {s.vx[i], s.vy[i], s.vz[i]};
+            advance_body(nx, i, dt, 1.0 / (dt * dt), 1.0 / (dt * dt * dt), f, p, u);
+            if constexpr (MIXED)
+                static_cast<JBMX*>(nx.rec)[i] = make_jbmx(p[0], p[1], p[2], s.m[i], u[0], u[1], u[2], vr);
+            else
+                static_cast<JB64*>(nx.rec)[i] = make_jb64(p[0], p[1], p[2], s.m[i], u[0], u[1], u[2]);
+        } else {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) out.c[c][it.i0 + t] = f[c];
+            for (int c = 0; c < 6; ++c) out.c[c][it.i0 + t] = f[c];
+        }
     }
 }
 
@@ -457,12 +577,26 @@ struct nbh_ensemble {
 
     // device
     DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot, symitems, vref;
-    DevBuf bigitems, biglist, rec;  // scalar-load kernel: items, its systems, packed records
+    // scalar-load kernel: items, its systems, and two buffers of packed
+    // records (one per prediction buffer), each indexed like the bodies
+    DevBuf bigitems, biglist, rec;
+    DevBuf bodysys;  // [total] the body's system if that is on the scalar-load kernel, else -1
+    // two buffers of the predicted state (x, y, z, vx, vy, vz; masses are the
+    // state's): pred[parity] holds this step's prediction, the step writes
+    // the next one into pred[1 - parity]
+    DevBuf pred;
+    int parity = 0;
+    int nsmall = 0;  // bodies of systems not on the scalar-load kernel
     int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0, nsym = 0, nbig = 0, nbigsys = 0;
     int list_ipl = 1;  // targets per lane of the uploaded work list
     bool uploaded = false, dt_set = false;
+    // pred[parity] and its records hold the prediction of the next step from
+    // the current state, f0, dt and the reference velocities (cleared by
+    // anything that changes one of them, or the records' coverage)
+    bool pred_valid = false;
     hipGraphExec_t gexec = nullptr;
     int g_steps = 0;
+    int g_parity = 0;  // parity the captured steps start from (and end at)
     // second stream (default priority) on which the persistent launch runs
     // beside the scalar-load launch, forked from and joined to the
     // context's stream with events (captured into the step graph as two
@@ -601,15 +735,30 @@ int upload_items(nbh_ensemble* e, const WorkList& w, int ipl)
     if (!w.big.empty()) {
         if (e->bigitems.reserve(sizeof(EnsBig) * w.big.size()) != hipSuccess ||
             e->biglist.reserve(sizeof(int) * w.bigsys.size()) != hipSuccess ||
-            e->rec.reserve(sizeof(nbh::JB64) * (size_t)e->total) != hipSuccess)
+            e->rec.reserve(2 * sizeof(nbh::JB64) * (size_t)e->total) != hipSuccess)
             return fail(NBH_ERR_ALLOC, "ensemble scalar-load buffers allocation failed");
         NBH_HIP(hipMemcpyAsync(e->bigitems.p, w.big.data(), sizeof(EnsBig) * w.big.size(), hipMemcpyHostToDevice,
                                c->stream));
         NBH_HIP(hipMemcpyAsync(e->biglist.p, w.bigsys.data(), sizeof(int) * w.bigsys.size(),
                                hipMemcpyHostToDevice, c->stream));
     }
+    {
+        std::vector<int> bsys(e->total, -1);
+        int nbigbodies = 0;
+        for (int k : w.bigsys) {
+            std::fill(bsys.begin() + e->off[k], bsys.begin() + e->off[k] + e->cnt[k], k);
+            nbigbodies += e->cnt[k];
+        }
+        if (e->bodysys.reserve(sizeof(int) * (size_t)e->total) != hipSuccess)
+            return fail(NBH_ERR_ALLOC, "ensemble body index allocation failed");
+        NBH_HIP(hipMemcpyAsync(e->bodysys.p, bsys.data(), sizeof(int) * (size_t)e->total, hipMemcpyHostToDevice,
+                               c->stream));
+        NBH_HIP(hipStreamSynchronize(c->stream));
+        e->nsmall = e->total - nbigbodies;
+    }
     e->nbig = (int)w.big.size();
     e->nbigsys = (int)w.bigsys.size();
+    e->pred_valid = false;
     if (e->symitems.reserve(sizeof(nbh::SymItem) * std::max<size_t>(sym.size(), 1)) != hipSuccess ||
         e->items.reserve(sizeof(EnsItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
         e->reduces.reserve(sizeof(EnsReduce) * std::max<size_t>(reds.size(), 1)) != hipSuccess ||
@@ -648,12 +797,23 @@ int rebuild(nbh_ensemble* e)
     return upload_items(e, w, e->ipl);
 }
 
-// Force evaluation of the current work list: state -> out.
-int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events)
+// Record buffer q (0 or 1) of the scalar-load kernel.
+void* rec_of(const nbh_ensemble* e, int q)
+{
+    return static_cast<char*>(e->rec.p) + (size_t)q * sizeof(nbh::JB64) * (size_t)e->total;
+}
+
+// Force evaluation of the current work list: src -> out.  prep: first the
+// reference velocities (mixed) and the scalar-load kernel's records of src;
+// without it they must already be there (the fused step kernel wrote them
+// for the predicted state).
+int enqueue_ens_force(nbh_ensemble* e, const nbh::State& src, const nbh::Forces& out, bool prep,
+                      bool allow_events, const nbh::EnsNext* nx = nullptr)
 {
     nbh_ctx* c = e->ctx;
+    void* rec = e->nbig > 0 ? rec_of(e, e->parity) : nullptr;
     nbh::EnsArgs a;
-    a.s = nbh::state_of(e->state.d(), e->total);
+    a.s = src;
     a.out = out;
     a.sys = static_cast<const nbh::EnsSys*>(e->sys.p);
     a.items = static_cast<const EnsItem*>(e->items.p);
@@ -670,9 +830,12 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
     }
     const bool mixed = c->cfg.precision == NBH_MIXED;
     const bool mask = e->any_mask[mixed];
-    if (mixed) {
-        nbh::k_ens_vref<<<e->nsys, nbh::kLanes, 0, c->stream>>>(a.s, a.sys, e->nsys, e->vref.d());
-        NBH_HIP(hipGetLastError());
+    if (prep) {
+        e->pred_valid = false;  // the reference velocities and records now describe src
+        if (mixed) {
+            nbh::k_ens_vref<<<e->nsys, nbh::kLanes, 0, c->stream>>>(a.s, a.sys, e->nsys, e->vref.d());
+            NBH_HIP(hipGetLastError());
+        }
     }
     // with both launches present, the persistent one runs on the side
     // stream, so that its items fill the scalar-load launch's tail
@@ -696,21 +859,34 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
     }
     if (e->nbig > 0) {  // the large systems' source records, then their scalar-load kernel
         const int* list = static_cast<const int*>(e->biglist.p);
-        if (mixed)
-            nbh::k_ens_pack<true><<<e->nbigsys, nbh::kBigBlock, 0, c->stream>>>(a.s, a.sys, list, a.vref, e->rec.p);
-        else
-            nbh::k_ens_pack<false><<<e->nbigsys, nbh::kBigBlock, 0, c->stream>>>(a.s, a.sys, list, a.vref, e->rec.p);
-        NBH_HIP(hipGetLastError());
+        if (prep) {
+            if (mixed)
+                nbh::k_ens_pack<true><<<e->nbigsys, nbh::kBigBlock, 0, c->stream>>>(a.s, a.sys, list, a.vref, rec);
+            else
+                nbh::k_ens_pack<false><<<e->nbigsys, nbh::kBigBlock, 0, c->stream>>>(a.s, a.sys, list, a.vref, rec);
+            NBH_HIP(hipGetLastError());
+        }
         const auto* big = static_cast<const EnsBig*>(e->bigitems.p);
         const dim3 g(e->nbig), b(nbh::kBigBlock);
-        if (mixed && mask)
-            nbh::k_ens_big<true, true><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
+        const nbh::EnsNext none{};
+        const nbh::EnsNext& n = nx ? *nx : none;
+        // <MIXED, MASK, ADVANCE>
+        if (mixed && mask && nx)
+            nbh::k_ens_big<true, true, true><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, rec, a.vref, n);
+        else if (mixed && mask)
+            nbh::k_ens_big<true, true, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, rec, a.vref, n);
+        else if (mixed && nx)
+            nbh::k_ens_big<true, false, true><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, rec, a.vref, n);
         else if (mixed)
-            nbh::k_ens_big<true, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
+            nbh::k_ens_big<true, false, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, rec, a.vref, n);
+        else if (mask && nx)
+            nbh::k_ens_big<false, true, true><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, rec, a.vref, n);
         else if (mask)
-            nbh::k_ens_big<false, true><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
+            nbh::k_ens_big<false, true, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, rec, a.vref, n);
+        else if (nx)
+            nbh::k_ens_big<false, false, true><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, rec, a.vref, n);
         else
-            nbh::k_ens_big<false, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, e->rec.p, a.vref);
+            nbh::k_ens_big<false, false, false><<<g, b, 0, c->stream>>>(a.s, out, a.sys, big, rec, a.vref, n);
         NBH_HIP(hipGetLastError());
     }
     if (fork) {
@@ -727,20 +903,84 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::Forces& out, bool allow_events
     return NBH_OK;
 }
 
+nbh::State current_state(const nbh_ensemble* e) { return nbh::state_of(e->state.d(), e->total); }
+
+// Prediction buffer q (0 or 1) as a state (masses: the current state's).
+nbh::State predicted_state(const nbh_ensemble* e, int q)
+{
+    const size_t n = e->total;
+    double* p = e->pred.d() + (size_t)q * 6 * n;
+    nbh::State s = current_state(e);
+    s.x = p;
+    s.y = p + n;
+    s.z = p + 2 * n;
+    s.vx = p + 3 * n;
+    s.vy = p + 4 * n;
+    s.vz = p + 5 * n;
+    return s;
+}
+
+// Where the step that reads prediction buffer `from` puts the corrected
+// state (the current state), the forces and the next prediction (buffer
+// `to`, with its records).
+nbh::EnsNext next_of(const nbh_ensemble* e, int to)
+{
+    nbh::EnsNext nx;
+    const size_t n = e->total;
+    double* b = e->state.d();
+    const nbh::State p = predicted_state(e, to);
+    for (int k = 0; k < 3; ++k) {
+        nx.cp[k] = b + k * n;
+        nx.cv[k] = b + (4 + k) * n;
+    }
+    nx.np[0] = const_cast<double*>(p.x);
+    nx.np[1] = const_cast<double*>(p.y);
+    nx.np[2] = const_cast<double*>(p.z);
+    nx.nv[0] = const_cast<double*>(p.vx);
+    nx.nv[1] = const_cast<double*>(p.vy);
+    nx.nv[2] = const_cast<double*>(p.vz);
+    nx.m = b + 3 * n;
+    nx.f0 = nbh::forces_of(e->f0.d(), e->total);
+    nx.dt = e->dt.d();
+    nx.rec = e->nbig > 0 ? rec_of(e, to) : nullptr;
+    return nx;
+}
+
+// First prediction of a run: the current state and f0 into pred[parity] and
+// its records.
+int enqueue_ens_prologue(nbh_ensemble* e)
+{
+    nbh_ctx* c = e->ctx;
+    const nbh::EnsNext nx = next_of(e, e->parity);
+    const int* bs = static_cast<const int*>(e->bodysys.p);
+    const dim3 g((e->total + nbh::kAdvBlock - 1) / nbh::kAdvBlock), b(nbh::kAdvBlock);
+    if (c->cfg.precision == NBH_MIXED)
+        nbh::k_ens_prologue<true><<<g, b, 0, c->stream>>>(nx, bs, e->vref.d(), e->total);
+    else
+        nbh::k_ens_prologue<false><<<g, b, 0, c->stream>>>(nx, bs, e->vref.d(), e->total);
+    NBH_HIP(hipGetLastError());
+    return NBH_OK;
+}
+
+// One shared-dt Hermite step (src/nbody.cpp:224-289) with the prediction in
+// pred[parity]: forces at the prediction; the scalar-load kernel corrects
+// and predicts its own targets, k_ens_advance every other body; then the
+// buffers swap roles.
 int enqueue_ens_step(nbh_ensemble* e, bool allow_events)
 {
     nbh_ctx* c = e->ctx;
-    const int n = e->total;
-    double* b = e->state.d();
-    const nbh::Forces f0 = nbh::forces_of(e->f0.d(), n);
-    const nbh::Forces f1 = nbh::forces_of(e->f1.d(), n);
-    double* dt = e->dt.d();
-    double *x = b, *y = b + n, *z = b + 2 * (size_t)n;
-    double *vx = b + 4 * (size_t)n, *vy = b + 5 * (size_t)n, *vz = b + 6 * (size_t)n;
-    NBH_HIP(nbh::launch_predict(n, dt, x, y, z, vx, vy, vz, f0, c->stream));
-    int rc = enqueue_ens_force(e, f1, allow_events);
+    const int p = e->parity;
+    const nbh::EnsNext nx = next_of(e, 1 - p);
+    const nbh::State pred = predicted_state(e, p);
+    const nbh::Forces f1 = nbh::forces_of(e->f1.d(), e->total);
+    int rc = enqueue_ens_force(e, pred, f1, false, allow_events, e->nbig > 0 ? &nx : nullptr);
     if (rc) return rc;
-    NBH_HIP(nbh::launch_correct(n, dt, x, y, z, vx, vy, vz, f0, f1, nullptr, 0.0, c->stream));
+    if (e->nsmall > 0) {
+        const dim3 g((e->total + nbh::kAdvBlock - 1) / nbh::kAdvBlock), b(nbh::kAdvBlock);
+        nbh::k_ens_advance<<<g, b, 0, c->stream>>>(nx, pred, f1, static_cast<const int*>(e->bodysys.p), e->total);
+        NBH_HIP(hipGetLastError());
+    }
+    e->parity = 1 - p;
     return NBH_OK;
 }
 
@@ -791,7 +1031,7 @@ int nbh_ens_create(nbh_ctx* ctx, int nsys, const int* counts, const double* eps2
     }
     const size_t nb = sizeof(double) * (size_t)total;
     if (e->sys.reserve(sizeof(nbh::EnsSys) * nsys) != hipSuccess || e->state.reserve(7 * nb) != hipSuccess ||
-        e->f0.reserve(6 * nb) != hipSuccess || e->f1.reserve(6 * nb) != hipSuccess ||
+        e->f0.reserve(6 * nb) != hipSuccess || e->f1.reserve(6 * nb) != hipSuccess || e->pred.reserve(12 * nb) != hipSuccess ||
         e->counter.reserve(sizeof(unsigned)) != hipSuccess || e->dt.reserve(sizeof(double)) != hipSuccess ||
         e->vref.reserve(sizeof(double) * 3 * (size_t)nsys) != hipSuccess ||
         e->ekin.reserve(sizeof(double) * nsys) != hipSuccess || e->epot.reserve(sizeof(double) * nsys) != hipSuccess) {
@@ -823,7 +1063,8 @@ int nbh_ens_destroy(nbh_ensemble* e)
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     if (e->side) (void)hipStreamDestroy(e->side);
     for (DevBuf* b : {&e->sys, &e->state, &e->f0, &e->f1, &e->items, &e->reduces, &e->partial, &e->counter,
-                      &e->dt, &e->ekin, &e->epot, &e->symitems, &e->vref, &e->bigitems, &e->biglist, &e->rec})
+                      &e->dt, &e->ekin, &e->epot, &e->symitems, &e->vref, &e->bigitems, &e->biglist, &e->rec,
+                      &e->bodysys, &e->pred})
         b->release();
     delete e;
     return NBH_OK;
@@ -841,7 +1082,7 @@ int nbh_ens_upload(nbh_ensemble* e, const double* x, const double* y, const doub
     const size_t nb = sizeof(double) * (size_t)e->total;
     for (int k = 0; k < 7; ++k)
         NBH_HIP(hipMemcpyAsync(e->state.d() + (size_t)k * e->total, ins[k], nb, hipMemcpyHostToDevice, c->stream));
-    int rc = enqueue_ens_force(e, nbh::forces_of(e->f0.d(), e->total), false);
+    int rc = enqueue_ens_force(e, current_state(e), nbh::forces_of(e->f0.d(), e->total), true, false);
     if (rc) return rc;
     NBH_HIP(hipStreamSynchronize(c->stream));
     e->uploaded = true;
@@ -855,6 +1096,7 @@ int nbh_ens_set_dt(nbh_ensemble* e, double dt)
     NBH_HIP(hipSetDevice(e->ctx->cfg.device));
     NBH_HIP(hipMemcpy(e->dt.p, &dt, sizeof dt, hipMemcpyHostToDevice));
     e->dt_set = true;
+    e->pred_valid = false;
     return NBH_OK;
 }
 
@@ -867,14 +1109,23 @@ int nbh_ens_step(nbh_ensemble* e, int nsteps, int use_graph)
     nbh_ctx* c = e->ctx;
     NBH_HIP(hipSetDevice(c->cfg.device));
     const auto t0 = std::chrono::steady_clock::now();
+    if (nsteps > 0 && !e->pred_valid) {  // the run's first prediction, outside any graph
+        int rc = enqueue_ens_prologue(e);
+        if (rc) return rc;
+        e->pred_valid = true;
+    }
     int done = 0;
     if (use_graph && nsteps > 0) {
         const int chunk = std::min(nsteps, 16);
-        if (!e->gexec || e->g_steps != chunk) {
+        // a captured sequence starts from the prediction buffer it was
+        // captured at (an even number of steps ends there too)
+        if (!e->gexec || e->g_steps != chunk || e->g_parity != e->parity) {
             drop_graph(e);
+            const int p0 = e->parity;
             NBH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int rc = NBH_OK;
             for (int k = 0; k < chunk && rc == NBH_OK; ++k) rc = enqueue_ens_step(e, false);
+            e->parity = p0;  // nothing has run yet
             hipGraph_t g = nullptr;
             hipError_t he = hipStreamEndCapture(c->stream, &g);
             if (rc) {
@@ -889,10 +1140,12 @@ int nbh_ens_step(nbh_ensemble* e, int nsteps, int use_graph)
                 return fail(NBH_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(he));
             }
             e->g_steps = chunk;
+            e->g_parity = p0;
         }
-        while (nsteps - done >= e->g_steps) {
+        while (nsteps - done >= e->g_steps && e->parity == e->g_parity) {
             NBH_HIP(hipGraphLaunch(e->gexec, c->stream));
             done += e->g_steps;
+            e->parity ^= e->g_steps & 1;
         }
     }
     for (; done < nsteps; ++done) {
@@ -983,7 +1236,7 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
         double mn = 1e300;
         for (int rep = 0; rep < 3; ++rep) {
             if (hipEventRecord(a, c->stream) != hipSuccess) return fail(NBH_ERR_HIP, "event record failed");
-            int r = enqueue_ens_force(e, nbh::forces_of(e->f1.d(), e->total), false);
+            int r = enqueue_ens_force(e, current_state(e), nbh::forces_of(e->f1.d(), e->total), true, false);
             if (r) return r;
             if (hipEventRecord(b, c->stream) != hipSuccess) return fail(NBH_ERR_HIP, "event record failed");
             if (hipEventSynchronize(b) != hipSuccess) return fail(NBH_ERR_HIP, "sync failed");

@@ -31,6 +31,7 @@
 // summed over all j != i instead of over the j < i triangle.
 #include "nbh_internal.h"
 #include "nbh_pair.h"
+#include "nbh_hermite.h"
 
 #include <algorithm>
 #include <cmath>
@@ -424,15 +425,12 @@ __global__ __launch_bounds__(kOBlock) void k_predict(int n, const double* __rest
     const int i = blockIdx.x * kOBlock + threadIdx.x;
     if (i >= n) return;
     const double dt = *dtp;
-    const double ax = f.c[0][i], ay = f.c[1][i], az = f.c[2][i];
-    const double jx = f.c[3][i], jy = f.c[4][i], jz = f.c[5][i];
-    const double ux = vx[i], uy = vy[i], uz = vz[i];
-    x[i] = x[i] + (ux * dt + 0.5 * ax * dt * dt + 1.0 / 6.0 * jx * dt * dt * dt);
-    y[i] = y[i] + (uy * dt + 0.5 * ay * dt * dt + 1.0 / 6.0 * jy * dt * dt * dt);
-    z[i] = z[i] + (uz * dt + 0.5 * az * dt * dt + 1.0 / 6.0 * jz * dt * dt * dt);
-    vx[i] = ux + (ax * dt + 0.5 * jx * dt * dt);
-    vy[i] = uy + (ay * dt + 0.5 * jy * dt * dt);
-    vz[i] = uz + (az * dt + 0.5 * jz * dt * dt);
+    double px = x[i], py = y[i], pz = z[i], ux = vx[i], uy = vy[i], uz = vz[i];
+    hermite_predict_axis(dt, f.c[0][i], f.c[3][i], px, ux);
+    hermite_predict_axis(dt, f.c[1][i], f.c[4][i], py, uy);
+    hermite_predict_axis(dt, f.c[2][i], f.c[5][i], pz, uz);
+    x[i] = px; y[i] = py; z[i] = pz;
+    vx[i] = ux; vy[i] = uy; vz[i] = uz;
 }
 
 __device__ __forceinline__ void block_min_u64(unsigned long long v, unsigned long long* out)
@@ -478,24 +476,17 @@ __global__ __launch_bounds__(kOBlock) void k_correct(int n, const double* __rest
         const double invdt2 = 1.0 / (dt * dt);
         const double invdt3 = 1.0 / (dt * dt * dt);
         double a2[3], a3[3], a0[3], j0[3];
+        double* const pos[3] = {x, y, z};
+        double* const vel[3] = {vx, vy, vz};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             a0[c] = f0.c[c][i];
             j0[c] = f0.c[3 + c][i];
-            const double a1 = f1.c[c][i];
-            const double j1 = f1.c[3 + c][i];
-            a2[c] = 2.0 * invdt2 * (-3.0 * (a0[c] - a1) - (2.0 * j0[c] + j1) * dt);
-            a3[c] = 6.0 * invdt3 * (2.0 * (a0[c] - a1) + (j0[c] + j1) * dt);
+            double p = pos[c][i], u = vel[c][i];
+            hermite_correct_axis(dt, invdt2, invdt3, a0[c], j0[c], f1.c[c][i], f1.c[3 + c][i], p, u, a2[c], a3[c]);
+            pos[c][i] = p;
+            vel[c][i] = u;
         }
-        const double dt3 = dt * dt * dt;
-        const double dt4 = dt3 * dt;
-        const double dt5 = dt4 * dt;
-        x[i] = x[i] + (1.0 / 24.0 * a2[0] * dt4 + 1.0 / 120.0 * a3[0] * dt5);
-        y[i] = y[i] + (1.0 / 24.0 * a2[1] * dt4 + 1.0 / 120.0 * a3[1] * dt5);
-        z[i] = z[i] + (1.0 / 24.0 * a2[2] * dt4 + 1.0 / 120.0 * a3[2] * dt5);
-        vx[i] = vx[i] + (1.0 / 6.0 * a2[0] * dt3 + 1.0 / 24.0 * a3[0] * dt4);
-        vy[i] = vy[i] + (1.0 / 6.0 * a2[1] * dt3 + 1.0 / 24.0 * a3[1] * dt4);
-        vz[i] = vz[i] + (1.0 / 6.0 * a2[2] * dt3 + 1.0 / 24.0 * a3[2] * dt4);
         if (dt_next) {
             const double na2 = a0[0] * a0[0] + a0[1] * a0[1] + a0[2] * a0[2];
             const double na = sqrt(na2);

@@ -40,20 +40,25 @@ __device__ __forceinline__ double rsqrt_f64(double s)
     return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
+__device__ __forceinline__ JB64 make_jb64(double x, double y, double z, double m, double vx, double vy, double vz)
+{
+    JB64 b;
+    b.x = x;
+    b.y = y;
+    b.z = z;
+    b.m = m;
+    b.vx = vx;
+    b.vy = vy;
+    b.vz = vz;
+    b.pad = 0.0;
+    return b;
+}
+
 // One source body gathered from the SoA state (coalesced when lanes take
 // consecutive j).
 __device__ __forceinline__ JB64 load_jb64(const State& s, int j)
 {
-    JB64 b;
-    b.x = s.x[j];
-    b.y = s.y[j];
-    b.z = s.z[j];
-    b.m = s.m[j];
-    b.vx = s.vx[j];
-    b.vy = s.vy[j];
-    b.vz = s.vz[j];
-    b.pad = 0.0;
-    return b;
+    return make_jb64(s.x[j], s.y[j], s.z[j], s.m[j], s.vx[j], s.vy[j], s.vz[j]);
 }
 
 // Reference velocity of a mixed-precision evaluation: every velocity is
@@ -72,18 +77,24 @@ __device__ __forceinline__ VRef load_vref(const double* __restrict__ p)
     return VRef{p[0], p[1], p[2]};
 }
 
-__device__ __forceinline__ JBMX load_jbmx(const State& s, int j, const VRef& vr)
+__device__ __forceinline__ JBMX make_jbmx(double x, double y, double z, double m, double vx, double vy, double vz,
+                                          const VRef& vr)
 {
     JBMX b;
-    b.vx = (float)(s.vx[j] - vr.x);
-    b.vy = (float)(s.vy[j] - vr.y);
-    b.vz = (float)(s.vz[j] - vr.z);
-    b.m = (float)s.m[j];
-    b.x = s.x[j];
-    b.y = s.y[j];
-    b.z = s.z[j];
+    b.vx = (float)(vx - vr.x);
+    b.vy = (float)(vy - vr.y);
+    b.vz = (float)(vz - vr.z);
+    b.m = (float)m;
+    b.x = x;
+    b.y = y;
+    b.z = z;
     b.pad = 0.0;
     return b;
+}
+
+__device__ __forceinline__ JBMX load_jbmx(const State& s, int j, const VRef& vr)
+{
+    return make_jbmx(s.x[j], s.y[j], s.z[j], s.m[j], s.vx[j], s.vy[j], s.vz[j], vr);
 }
 
 // ---------------------------------------------------------------- FP64 --

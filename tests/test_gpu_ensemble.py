@@ -372,3 +372,36 @@ def test_two_stream_step_graph_matches_eager(nb, small, precision):
             out[graph] = e.download()
     assert np.array_equal(out[True][0], out[False][0])
     assert np.array_equal(out[True][1], out[False][1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_step_paths_bitwise(nb, small, precision):
+    """A step corrects and predicts the scalar-load kernel's systems inside
+    that kernel (double-buffered predictions and records) and every other
+    system in a separate one-thread-per-body pass; the arithmetic is the
+    same, so five steps (odd: the run ends on the other prediction buffer)
+    give the same states and forces bit for bit with every bucket on the LDS
+    kernel, every bucket on the scalar-load kernel, or a mixture.  Steps
+    split across calls, eager and graph-replayed (a captured sequence
+    started from either buffer), equal the same number of eager steps."""
+    counts, eps2, state = small
+    dt = 1.0 / 256
+
+    def run(tiles, plan):
+        with nb.Backend(precision) as b, nb.Ensemble(b, counts, eps2) as e:
+            e.set_tuning(tiles, [1, 1, 1, 1])
+            e.upload(state)
+            e.set_dt(dt)
+            for n, graph in plan:
+                e.step(n, graph=graph)
+            return e.download()
+
+    five = [(5, False)]
+    ref = run([256] * 4, five)
+    for tiles in ([0] * 4, [64, 0, 256, 0]):
+        got = run(tiles, five)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), tiles
+    eager = run([64, 0, 256, 0], [(37, False)])
+    mixed_plan = run([64, 0, 256, 0], [(3, False), (16, True), (1, False), (16, True), (1, True)])
+    assert np.array_equal(mixed_plan[0], eager[0]) and np.array_equal(mixed_plan[1], eager[1])
+    _banner(f"{precision} ensemble step paths and graph/eager splits bitwise equal")
