@@ -68,6 +68,20 @@ struct EnsReduce {
     int splits;
 };
 
+// Where a step puts the corrected state, the forces and the next prediction
+// (the prediction buffer it does not read, with its records): see the step
+// kernels below.
+struct EnsNext {
+    double* cp[3];  // corrected positions (the current state)
+    double* cv[3];  // corrected velocities
+    double* np[3];  // next predicted positions
+    double* nv[3];  // next predicted velocities
+    const double* m;
+    Forces f0;      // forces at the start of the step, replaced by this step's
+    const double* dt;  // null: a force evaluation only, write `out`
+    void* rec;      // next records (scalar-load systems), indexed like the bodies
+};
+
 struct EnsArgs {
     State s;
     Forces out;
@@ -77,7 +91,33 @@ struct EnsArgs {
     unsigned* counter;
     double* partial;  // [slot][6][kMaxTargets]
     const double* vref;  // mixed: reference velocity of each system, [3 * nsys]
+    EnsNext nx;          // a step (nx.dt set): unsplit items of directed systems advance their targets
+    const int* bodysys;  // [total]: system on the scalar-load kernel, -1 other directed, -2 symmetric
 };
+
+// Corrector of one body from its prediction (p, u) and the forces f1 at it,
+// f0 <- f1, then the next prediction; returns the corrected state in (p, u)
+// replaced by the next prediction.
+__device__ __forceinline__ void advance_body(const EnsNext& nx, int i, double dt, double invdt2, double invdt3,
+                                             const double (&f1)[6], double (&p)[3], double (&u)[3])
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double a2, a3;
+        hermite_correct_axis(dt, invdt2, invdt3, nx.f0.c[c][i], nx.f0.c[3 + c][i], f1[c], f1[3 + c], p[c], u[c],
+                             a2, a3);
+        nx.cp[c][i] = p[c];
+        nx.cv[c][i] = u[c];
+        nx.f0.c[c][i] = f1[c];
+        nx.f0.c[3 + c][i] = f1[3 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        hermite_predict_axis(dt, f1[c], f1[3 + c], p[c], u[c]);
+        nx.np[c][i] = p[c];
+        nx.nv[c][i] = u[c];
+    }
+}
 
 int bucket_of(int n)
 {
@@ -205,11 +245,20 @@ __device__ __forceinline__ void ens_item(const EnsArgs& a, const EnsItem& it, JB
 #pragma unroll
             for (int c = 0; c < 6; ++c) out[q][c] = acc[q][c];
     }
+    // a step advances the targets of an unsplit item of a directed system
+    // (the forces are complete here); a symmetric system's diagonal block
+    // still gets the off-diagonal pairs added to `out`
+    const bool adv = a.nx.dt && it.slot < 0 && a.bodysys[it.i0] == -1;
+    const double dt = adv ? *a.nx.dt : 0.0;
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
         const int t = q * kLanes + lane;
         if (t < it.icount) {
-            if (it.slot < 0) {
+            if (adv) {
+                double p[3] = {xi[q], yi[q], zi[q]};
+                double u[3] = {a.s.vx[ti[q]], a.s.vy[ti[q]], a.s.vz[ti[q]]};
+                advance_body(a.nx, ti[q], dt, 1.0 / (dt * dt), 1.0 / (dt * dt * dt), out[q], p, u);
+            } else if (it.slot < 0) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) a.out.c[c][ti[q]] = out[q][c];
             } else {
@@ -235,17 +284,30 @@ __global__ __launch_bounds__(kLanes) void k_ens_force(EnsArgs a)
     }
 }
 
+// Split items' partial sums added in split order; in a step (nx.dt set),
+// the targets are then advanced (split items are always directed systems).
 __global__ __launch_bounds__(kLanes) void k_ens_reduce(const EnsReduce* r, int nr,
-                                                     const double* __restrict__ partial, Forces out)
+                                                     const double* __restrict__ partial, Forces out, State s,
+                                                     EnsNext nx)
 {
     const int b = blockIdx.x;
     if (b >= nr) return;
     const EnsReduce q = r[b];
     for (int t = threadIdx.x; t < q.icount; t += kLanes) {
+        double f[6];
         for (int c = 0; c < 6; ++c) {
             double v = 0.0;
-            for (int s = 0; s < q.splits; ++s) v += partial[((size_t)(q.slot0 + s) * 6 + c) * kMaxTargets + t];
-            out.c[c][q.i0 + t] = v;
+            for (int sp = 0; sp < q.splits; ++sp) v += partial[((size_t)(q.slot0 + sp) * 6 + c) * kMaxTargets + t];
+            f[c] = v;
+        }
+        const int i = q.i0 + t;
+        if (nx.dt) {
+            const double dt = *nx.dt;
+            double p[3] = {s.x[i], s.y[i], s.z[i]};
+            double u[3] = {s.vx[i], s.vy[i], s.vz[i]};
+            advance_body(nx, i, dt, 1.0 / (dt * dt), 1.0 / (dt * dt * dt), f, p, u);
+        } else {
+            for (int c = 0; c < 6; ++c) out.c[c][i] = f[c];
         }
     }
 }
@@ -327,50 +389,16 @@ __global__ __launch_bounds__(kBigBlock) void k_ens_pack(State s, const EnsSys* _
 // of the state the last force evaluation with preparation was made on
 // (upload, autotune): pairwise forces conserve it, so it stays the
 // centre-of-mass velocity to rounding, without a reduction per step.
-struct EnsNext {
-    double* cp[3];  // corrected positions (the current state)
-    double* cv[3];  // corrected velocities
-    double* np[3];  // next predicted positions
-    double* nv[3];  // next predicted velocities
-    const double* m;
-    Forces f0;      // forces at the start of the step, replaced by this step's
-    const double* dt;
-    void* rec;      // next records (scalar-load systems), indexed like the bodies
-};
-
-// Corrector of one body from its prediction (p, u) and the forces f1 at it,
-// f0 <- f1, then the next prediction; returns the corrected state in (p, u)
-// replaced by the next prediction.
-__device__ __forceinline__ void advance_body(const EnsNext& nx, int i, double dt, double invdt2, double invdt3,
-                                             const double (&f1)[6], double (&p)[3], double (&u)[3])
-{
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double a2, a3;
-        hermite_correct_axis(dt, invdt2, invdt3, nx.f0.c[c][i], nx.f0.c[3 + c][i], f1[c], f1[3 + c], p[c], u[c],
-                             a2, a3);
-        nx.cp[c][i] = p[c];
-        nx.cv[c][i] = u[c];
-        nx.f0.c[c][i] = f1[c];
-        nx.f0.c[3 + c][i] = f1[3 + c];
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        hermite_predict_axis(dt, f1[c], f1[3 + c], p[c], u[c]);
-        nx.np[c][i] = p[c];
-        nx.nv[c][i] = u[c];
-    }
-}
-
 constexpr int kAdvBlock = 256;
 
-// One thread per body of the systems not on the scalar-load kernel
-// (bodysys[i] < 0): the prediction (pp, pv) and the forces f1 at it.
+// One thread per body of the symmetric systems (bodysys[i] == -2), whose
+// forces are complete only after the symmetric kernel's atomic adds: the
+// prediction and the forces f1 at it.
 __global__ __launch_bounds__(kAdvBlock) void k_ens_advance(EnsNext nx, State pred, Forces f1,
                                                          const int* __restrict__ bodysys, int total)
 {
     const int i = blockIdx.x * kAdvBlock + threadIdx.x;
-    if (i >= total || bodysys[i] >= 0) return;
+    if (i >= total || bodysys[i] != -2) return;
     const double dt = *nx.dt;
     const double invdt2 = 1.0 / (dt * dt);
     const double invdt3 = 1.0 / (dt * dt * dt);
@@ -586,7 +614,7 @@ struct nbh_ensemble {
     // the next one into pred[1 - parity]
     DevBuf pred;
     int parity = 0;
-    int nsmall = 0;  // bodies of systems not on the scalar-load kernel
+    int nflat = 0;  // bodies of symmetric systems (advanced by k_ens_advance)
     int nitems = 0, nreduce = 0, nslots = 0, maxtile = 64, grid = 0, nsym = 0, nbig = 0, nbigsys = 0;
     int list_ipl = 1;  // targets per lane of the uploaded work list
     bool uploaded = false, dt_set = false;
@@ -632,6 +660,7 @@ struct WorkList {
     std::vector<nbh::SymItem> sym;
     std::vector<EnsBig> big;
     std::vector<int> bigsys;
+    std::vector<int> symsys;  // systems with symmetric pairing
     int nslots = 0;
     int maxtile() const
     {
@@ -670,6 +699,7 @@ void build_items(const nbh_ensemble* e, const int* tile, const int* split, int i
                 const int len = P < nr ? nbh::sym_list_len(nr, P, nb > nr) : 0;
                 if (len >= 1) sym.push_back({off, n, P, 0, len - 1, k, e->eps2[k]});
             }
+            w.symsys.push_back(k);
             continue;
         }
         if (T == 0) {
@@ -743,18 +773,21 @@ int upload_items(nbh_ensemble* e, const WorkList& w, int ipl)
                                hipMemcpyHostToDevice, c->stream));
     }
     {
+        // each body's system if that is on the scalar-load kernel, -2 for a
+        // symmetric system, -1 for any other
         std::vector<int> bsys(e->total, -1);
-        int nbigbodies = 0;
-        for (int k : w.bigsys) {
-            std::fill(bsys.begin() + e->off[k], bsys.begin() + e->off[k] + e->cnt[k], k);
-            nbigbodies += e->cnt[k];
+        for (int k : w.bigsys) std::fill(bsys.begin() + e->off[k], bsys.begin() + e->off[k] + e->cnt[k], k);
+        int nflat = 0;
+        for (int k : w.symsys) {
+            std::fill(bsys.begin() + e->off[k], bsys.begin() + e->off[k] + e->cnt[k], -2);
+            nflat += e->cnt[k];
         }
         if (e->bodysys.reserve(sizeof(int) * (size_t)e->total) != hipSuccess)
             return fail(NBH_ERR_ALLOC, "ensemble body index allocation failed");
         NBH_HIP(hipMemcpyAsync(e->bodysys.p, bsys.data(), sizeof(int) * (size_t)e->total, hipMemcpyHostToDevice,
                                c->stream));
         NBH_HIP(hipStreamSynchronize(c->stream));
-        e->nsmall = e->total - nbigbodies;
+        e->nflat = nflat;
     }
     e->nbig = (int)w.big.size();
     e->nbigsys = (int)w.bigsys.size();
@@ -820,6 +853,8 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::State& src, const nbh::Forces&
     a.nitems = e->nitems;
     a.counter = static_cast<unsigned*>(e->counter.p);
     a.partial = e->partial.d();
+    a.nx = nx ? *nx : nbh::EnsNext{};
+    a.bodysys = static_cast<const int*>(e->bodysys.p);
     a.vref = e->vref.d();
     nbh::EventGuard ev;
     const bool timed = c->profiling && allow_events;
@@ -854,7 +889,7 @@ int enqueue_ens_force(nbh_ensemble* e, const nbh::State& src, const nbh::Forces&
     NBH_HIP(hipLaunchKernel(k, dim3(e->grid), dim3(nbh::kLanes), args, 0, ps));
     if (e->nreduce > 0) {
         nbh::k_ens_reduce<<<e->nreduce, nbh::kLanes, 0, ps>>>(static_cast<const EnsReduce*>(e->reduces.p),
-                                                              e->nreduce, e->partial.d(), out);
+                                                              e->nreduce, e->partial.d(), out, a.s, a.nx);
         NBH_HIP(hipGetLastError());
     }
     if (e->nbig > 0) {  // the large systems' source records, then their scalar-load kernel
@@ -963,9 +998,10 @@ int enqueue_ens_prologue(nbh_ensemble* e)
 }
 
 // One shared-dt Hermite step (src/nbody.cpp:224-289) with the prediction in
-// pred[parity]: forces at the prediction; the scalar-load kernel corrects
-// and predicts its own targets, k_ens_advance every other body; then the
-// buffers swap roles.
+// pred[parity]: forces at the prediction, the force kernels correcting and
+// predicting the targets of the directed systems (the scalar-load kernel,
+// unsplit persistent items, the split reduction), k_ens_advance the
+// symmetric systems' bodies; then the buffers swap roles.
 int enqueue_ens_step(nbh_ensemble* e, bool allow_events)
 {
     nbh_ctx* c = e->ctx;
@@ -973,9 +1009,9 @@ int enqueue_ens_step(nbh_ensemble* e, bool allow_events)
     const nbh::EnsNext nx = next_of(e, 1 - p);
     const nbh::State pred = predicted_state(e, p);
     const nbh::Forces f1 = nbh::forces_of(e->f1.d(), e->total);
-    int rc = enqueue_ens_force(e, pred, f1, false, allow_events, e->nbig > 0 ? &nx : nullptr);
+    int rc = enqueue_ens_force(e, pred, f1, false, allow_events, &nx);
     if (rc) return rc;
-    if (e->nsmall > 0) {
+    if (e->nflat > 0) {
         const dim3 g((e->total + nbh::kAdvBlock - 1) / nbh::kAdvBlock), b(nbh::kAdvBlock);
         nbh::k_ens_advance<<<g, b, 0, c->stream>>>(nx, pred, f1, static_cast<const int*>(e->bodysys.p), e->total);
         NBH_HIP(hipGetLastError());
