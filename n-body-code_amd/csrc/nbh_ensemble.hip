@@ -380,7 +380,10 @@ __global__ __launch_bounds__(kBigBlock) void k_ens_pack(State s, const EnsSys* _
 //     workgroup, once its targets' forces are summed, runs the corrector,
 //     f0 <- f1 and the next prediction for those targets, and packs their
 //     next records: no pass over memory of its own.
-//   * Every other system: k_ens_advance, one thread per body.
+//   * Other directed systems likewise in the persistent kernel's unsplit
+//     items (ens_item) and, for split ones, in k_ens_reduce.
+//   * Symmetric systems, whose forces are complete only after the symmetric
+//     kernel: k_ens_advance, one thread per body.
 //   * The first step of a run: k_ens_prologue predicts from the current
 //     state and f0, and packs the records.
 // The arithmetic is the single-system kernels' (nbh_hermite.h), so the
@@ -608,7 +611,9 @@ struct nbh_ensemble {
     // scalar-load kernel: items, its systems, and two buffers of packed
     // records (one per prediction buffer), each indexed like the bodies
     DevBuf bigitems, biglist, rec;
-    DevBuf bodysys;  // [total] the body's system if that is on the scalar-load kernel, else -1
+    // [total] the body's system if that is on the scalar-load kernel, -2 if
+    // it is symmetric, -1 for any other directed system
+    DevBuf bodysys;
     // two buffers of the predicted state (x, y, z, vx, vy, vz; masses are the
     // state's): pred[parity] holds this step's prediction, the step writes
     // the next one into pred[1 - parity]
@@ -838,8 +843,10 @@ void* rec_of(const nbh_ensemble* e, int q)
 
 // Force evaluation of the current work list: src -> out.  prep: first the
 // reference velocities (mixed) and the scalar-load kernel's records of src;
-// without it they must already be there (the fused step kernel wrote them
-// for the predicted state).
+// without it they must already be there (written for the prediction by the
+// previous step or the prologue).  nx (a step): the directed systems'
+// targets are advanced into nx by the force kernels instead of written to
+// out.
 int enqueue_ens_force(nbh_ensemble* e, const nbh::State& src, const nbh::Forces& out, bool prep,
                       bool allow_events, const nbh::EnsNext* nx = nullptr)
 {
