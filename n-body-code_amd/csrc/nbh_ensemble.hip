@@ -970,17 +970,13 @@ nbh::EnsNext next_of(const nbh_ensemble* e, int to)
     nbh::EnsNext nx;
     const size_t n = e->total;
     double* b = e->state.d();
-    const nbh::State p = predicted_state(e, to);
+    double* p = e->pred.d() + (size_t)to * 6 * n;  // the layout of predicted_state
     for (int k = 0; k < 3; ++k) {
         nx.cp[k] = b + k * n;
         nx.cv[k] = b + (4 + k) * n;
+        nx.np[k] = p + k * n;
+        nx.nv[k] = p + (3 + k) * n;
     }
-    nx.np[0] = const_cast<double*>(p.x);
-    nx.np[1] = const_cast<double*>(p.y);
-    nx.np[2] = const_cast<double*>(p.z);
-    nx.nv[0] = const_cast<double*>(p.vx);
-    nx.nv[1] = const_cast<double*>(p.vy);
-    nx.nv[2] = const_cast<double*>(p.vz);
     nx.m = b + 3 * n;
     nx.f0 = nbh::forces_of(e->f0.d(), e->total);
     nx.dt = e->dt.d();
