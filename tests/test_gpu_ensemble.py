@@ -405,3 +405,32 @@ def test_step_paths_bitwise(nb, small, precision):
     mixed_plan = run([64, 0, 256, 0], [(3, False), (16, True), (1, False), (16, True), (1, True)])
     assert np.array_equal(mixed_plan[0], eager[0]) and np.array_equal(mixed_plan[1], eager[1])
     _banner(f"{precision} ensemble step paths and graph/eager splits bitwise equal")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_zero_softening_steps(nb, oracle, precision):
+    """Systems with eps2 = 0 take the kernels that drop the self pair; their
+    steps advance the targets inside those kernels too.  Three steps with
+    every bucket on the scalar-load kernel equal three steps on the LDS
+    kernel bit for bit, and (FP64) the oracle's run of each system."""
+    counts = np.array([50, 130, 300], dtype=np.int32)
+    eps2 = np.array([0.0, 1e-3, 0.0])
+    state = nb.ensemble_state(counts, 5)
+    dt, nsteps = 1.0 / 256, 3
+    out = {}
+    for tiles in ([0] * 4, [64] * 4):
+        with nb.Backend(precision) as b, nb.Ensemble(b, counts, eps2) as e:
+            e.set_tuning(tiles, [1] * 4)
+            e.upload(state)
+            e.set_dt(dt)
+            e.step(nsteps)
+            out[tiles[0]] = e.download()
+    assert np.array_equal(out[0][0], out[64][0]) and np.array_equal(out[0][1], out[64][1])
+    if precision == "fp64":
+        st = out[0][0]
+        o = 0
+        for k, n in enumerate(counts):
+            ref = oracle.run(state[:, o:o + n], eps2[k], dt, nsteps)
+            assert np.abs(st[:, o:o + n] - ref["state"]).max() <= 1e-12
+            o += n
+    _banner(f"{precision} zero-softening ensemble steps on both kernels")
