@@ -145,6 +145,12 @@ def test_unsupported_n_runs_directed(nb, precision):
         assert shape["pairing"] == "directed"
         fd, _ = force(nb, s, 1e-4, "directed", precision)
         assert np.array_equal(fs, fd)
+    # the symmetric kernels address the outputs with 32-bit byte offsets, so
+    # they take n <= 2^29; above that the directed kernel is selected (the
+    # launch-shape query allocates nothing)
+    with nb.Backend(precision, 1e-4, pairing="symmetric") as b:
+        assert b.launch_shape(1 << 29)["pairing"] == "symmetric"
+        assert b.launch_shape((1 << 29) + 1)["pairing"] == "directed"
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
