@@ -2,9 +2,10 @@
 //
 // Layout: the systems' SoA arrays are concatenated in system order (system k
 // occupies [off_k, off_k + n_k)).  A step is the force evaluation at the
-// predicted state, then one fused kernel per system (k_ens_advance): the
-// corrector, f0 <- f1, the next prediction, the reference velocity and the
-// source records, with the single-system kernels' arithmetic.
+// predicted state, whose kernels also run the corrector and the next
+// prediction (and its source records) for the targets they complete, with
+// the single-system kernels' arithmetic; predictions, records and forces
+// alternate between two buffers from step to step.
 //
 // Force evaluation: one persistent launch of one-wave (64-lane) workgroups.
 // Each wave takes work items from a device queue with an atomic counter:
@@ -77,7 +78,8 @@ struct EnsNext {
     double* np[3];  // next predicted positions
     double* nv[3];  // next predicted velocities
     const double* m;
-    Forces f0;      // forces at the start of the step, replaced by this step's
+    Forces f0;      // forces at the start of the step (read)
+    Forces f1;      // this step's forces, at the prediction (written; the next step's f0)
     const double* dt;  // null: a force evaluation only, write `out`
     void* rec;      // next records (scalar-load systems), indexed like the bodies
 };
@@ -95,9 +97,11 @@ struct EnsArgs {
     const int* bodysys;  // [total]: system on the scalar-load kernel, -1 other directed, -2 symmetric
 };
 
-// Corrector of one body from its prediction (p, u) and the forces f1 at it,
-// f0 <- f1, then the next prediction; returns the corrected state in (p, u)
-// replaced by the next prediction.
+// Corrector of one body from its prediction (p, u) and this step's forces
+// f1 at it (stored into nx.f1 with STORE, else already there), then the next
+// prediction; returns the corrected state in (p, u) replaced by the next
+// prediction.
+template <bool STORE = true>
 __device__ __forceinline__ void advance_body(const EnsNext& nx, int i, double dt, double invdt2, double invdt3,
                                              const double (&f1)[6], double (&p)[3], double (&u)[3])
 {
@@ -108,8 +112,10 @@ __device__ __forceinline__ void advance_body(const EnsNext& nx, int i, double dt
                              a2, a3);
         nx.cp[c][i] = p[c];
         nx.cv[c][i] = u[c];
-        nx.f0.c[c][i] = f1[c];
-        nx.f0.c[3 + c][i] = f1[3 + c];
+        if constexpr (STORE) {
+            nx.f1.c[c][i] = f1[c];
+            nx.f1.c[3 + c][i] = f1[3 + c];
+        }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -372,13 +378,13 @@ __global__ __launch_bounds__(kBigBlock) void k_ens_pack(State s, const EnsSys* _
 
 // The ensemble's step after its force evaluation.  The prediction of a step
 // is double-buffered (pred[0], pred[1], and likewise the records of the
-// scalar-load kernel's systems): step n reads buffer p and writes the next
-// prediction into 1 - p, so a kernel can write the next prediction of its
-// own targets while other workgroups of the same system still read this
-// step's sources.
+// scalar-load kernel's systems and the forces): step n reads buffer p and
+// writes its forces and the next prediction into 1 - p, so a kernel can
+// write the next prediction of its own targets while other workgroups of
+// the same system still read this step's sources, and no forces are copied.
 //   * Systems on the scalar-load kernel (k_ens_big with ADVANCE): each
-//     workgroup, once its targets' forces are summed, runs the corrector,
-//     f0 <- f1 and the next prediction for those targets, and packs their
+//     workgroup, once its targets' forces are summed, stores them, runs the
+//     corrector and the next prediction for those targets, and packs their
 //     next records: no pass over memory of its own.
 //   * Other directed systems likewise in the persistent kernel's unsplit
 //     items (ens_item) and, for split ones, in k_ens_reduce.
@@ -396,7 +402,7 @@ constexpr int kAdvBlock = 256;
 
 // One thread per body of the symmetric systems (bodysys[i] == -2), whose
 // forces are complete only after the symmetric kernel's atomic adds: the
-// prediction and the forces f1 at it.
+// prediction and this step's forces f1 at it (already in nx.f1).
 __global__ __launch_bounds__(kAdvBlock) void k_ens_advance(EnsNext nx, State pred, Forces f1,
                                                          const int* __restrict__ bodysys, int total)
 {
@@ -410,7 +416,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_ens_advance(EnsNext nx, State pre
     for (int c = 0; c < 6; ++c) f[c] = f1.c[c][i];
     double p[3] = {pred.x[i], pred.y[i], pred.z[i]};
     double u[3] = {pred.vx[i], pred.vy[i], pred.vz[i]};
-    advance_body(nx, i, dt, invdt2, invdt3, f, p, u);
+    advance_body<false>(nx, i, dt, invdt2, invdt3, f, p, u);
 }
 
 // First prediction of a run, one thread per body: from the current state
@@ -607,7 +613,10 @@ struct nbh_ensemble {
     bool sym = false;  // systems of >= 2 symmetric blocks use symmetric pairing
 
     // device
-    DevBuf sys, state, f0, f1, items, reduces, partial, counter, dt, ekin, epot, symitems, vref;
+    // force[parity] holds the forces at the current state; a step writes its
+    // forces, at the prediction, into force[1 - parity]
+    DevBuf force[2];
+    DevBuf sys, state, items, reduces, partial, counter, dt, ekin, epot, symitems, vref;
     // scalar-load kernel: items, its systems, and two buffers of packed
     // records (one per prediction buffer), each indexed like the bodies
     DevBuf bigitems, biglist, rec;
@@ -962,10 +971,13 @@ nbh::State predicted_state(const nbh_ensemble* e, int q)
     return s;
 }
 
-// Where the step that reads prediction buffer `from` puts the corrected
-// state (the current state), the forces and the next prediction (buffer
-// `to`, with its records).
-nbh::EnsNext next_of(const nbh_ensemble* e, int to)
+// Force buffer q (0 or 1).
+nbh::Forces forces_buf(const nbh_ensemble* e, int q) { return nbh::forces_of(e->force[q].d(), e->total); }
+
+// A step that starts from buffers `from` (prediction, forces at the current
+// state) reads f0 there and puts the corrected state (the current state),
+// its forces and the next prediction with its records into buffers `to`.
+nbh::EnsNext next_of(const nbh_ensemble* e, int from, int to)
 {
     nbh::EnsNext nx;
     const size_t n = e->total;
@@ -978,18 +990,19 @@ nbh::EnsNext next_of(const nbh_ensemble* e, int to)
         nx.nv[k] = p + (3 + k) * n;
     }
     nx.m = b + 3 * n;
-    nx.f0 = nbh::forces_of(e->f0.d(), e->total);
+    nx.f0 = forces_buf(e, from);
+    nx.f1 = forces_buf(e, to);
     nx.dt = e->dt.d();
     nx.rec = e->nbig > 0 ? rec_of(e, to) : nullptr;
     return nx;
 }
 
-// First prediction of a run: the current state and f0 into pred[parity] and
-// its records.
+// First prediction of a run: the current state and its forces
+// (force[parity]) into pred[parity] and its records.
 int enqueue_ens_prologue(nbh_ensemble* e)
 {
     nbh_ctx* c = e->ctx;
-    const nbh::EnsNext nx = next_of(e, e->parity);
+    const nbh::EnsNext nx = next_of(e, e->parity, e->parity);
     const int* bs = static_cast<const int*>(e->bodysys.p);
     const dim3 g((e->total + nbh::kAdvBlock - 1) / nbh::kAdvBlock), b(nbh::kAdvBlock);
     if (c->cfg.precision == NBH_MIXED)
@@ -1001,17 +1014,18 @@ int enqueue_ens_prologue(nbh_ensemble* e)
 }
 
 // One shared-dt Hermite step (src/nbody.cpp:224-289) with the prediction in
-// pred[parity]: forces at the prediction, the force kernels correcting and
-// predicting the targets of the directed systems (the scalar-load kernel,
-// unsplit persistent items, the split reduction), k_ens_advance the
-// symmetric systems' bodies; then the buffers swap roles.
+// pred[parity]: forces at the prediction into force[1 - parity], the force
+// kernels correcting and predicting the targets of the directed systems (the
+// scalar-load kernel, unsplit persistent items, the split reduction),
+// k_ens_advance the symmetric systems' bodies; then the buffers swap roles
+// (the reference swaps its force pointers the same way, src/nbody.cpp:263-288).
 int enqueue_ens_step(nbh_ensemble* e, bool allow_events)
 {
     nbh_ctx* c = e->ctx;
     const int p = e->parity;
-    const nbh::EnsNext nx = next_of(e, 1 - p);
+    const nbh::EnsNext nx = next_of(e, p, 1 - p);
     const nbh::State pred = predicted_state(e, p);
-    const nbh::Forces f1 = nbh::forces_of(e->f1.d(), e->total);
+    const nbh::Forces f1 = forces_buf(e, 1 - p);
     int rc = enqueue_ens_force(e, pred, f1, false, allow_events, &nx);
     if (rc) return rc;
     if (e->nflat > 0) {
@@ -1070,7 +1084,8 @@ int nbh_ens_create(nbh_ctx* ctx, int nsys, const int* counts, const double* eps2
     }
     const size_t nb = sizeof(double) * (size_t)total;
     if (e->sys.reserve(sizeof(nbh::EnsSys) * nsys) != hipSuccess || e->state.reserve(7 * nb) != hipSuccess ||
-        e->f0.reserve(6 * nb) != hipSuccess || e->f1.reserve(6 * nb) != hipSuccess || e->pred.reserve(12 * nb) != hipSuccess ||
+        e->force[0].reserve(6 * nb) != hipSuccess || e->force[1].reserve(6 * nb) != hipSuccess ||
+        e->pred.reserve(12 * nb) != hipSuccess ||
         e->counter.reserve(sizeof(unsigned)) != hipSuccess || e->dt.reserve(sizeof(double)) != hipSuccess ||
         e->vref.reserve(sizeof(double) * 3 * (size_t)nsys) != hipSuccess ||
         e->ekin.reserve(sizeof(double) * nsys) != hipSuccess || e->epot.reserve(sizeof(double) * nsys) != hipSuccess) {
@@ -1101,7 +1116,7 @@ int nbh_ens_destroy(nbh_ensemble* e)
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     if (e->side) (void)hipStreamDestroy(e->side);
-    for (DevBuf* b : {&e->sys, &e->state, &e->f0, &e->f1, &e->items, &e->reduces, &e->partial, &e->counter,
+    for (DevBuf* b : {&e->sys, &e->state, &e->force[0], &e->force[1], &e->items, &e->reduces, &e->partial, &e->counter,
                       &e->dt, &e->ekin, &e->epot, &e->symitems, &e->vref, &e->bigitems, &e->biglist, &e->rec,
                       &e->bodysys, &e->pred})
         b->release();
@@ -1121,7 +1136,7 @@ int nbh_ens_upload(nbh_ensemble* e, const double* x, const double* y, const doub
     const size_t nb = sizeof(double) * (size_t)e->total;
     for (int k = 0; k < 7; ++k)
         NBH_HIP(hipMemcpyAsync(e->state.d() + (size_t)k * e->total, ins[k], nb, hipMemcpyHostToDevice, c->stream));
-    int rc = enqueue_ens_force(e, current_state(e), nbh::forces_of(e->f0.d(), e->total), true, false);
+    int rc = enqueue_ens_force(e, current_state(e), forces_buf(e, e->parity), true, false);
     if (rc) return rc;
     NBH_HIP(hipStreamSynchronize(c->stream));
     e->uploaded = true;
@@ -1213,7 +1228,8 @@ int nbh_ens_download(nbh_ensemble* e, double* x, double* y, double* z, double* m
             NBH_HIP(hipMemcpyAsync(so[k], e->state.d() + (size_t)k * e->total, nb, hipMemcpyDeviceToHost, c->stream));
     for (int k = 0; k < 6; ++k)
         if (fo[k])
-            NBH_HIP(hipMemcpyAsync(fo[k], e->f0.d() + (size_t)k * e->total, nb, hipMemcpyDeviceToHost, c->stream));
+            NBH_HIP(hipMemcpyAsync(fo[k], e->force[e->parity].d() + (size_t)k * e->total, nb,
+                                   hipMemcpyDeviceToHost, c->stream));
     NBH_HIP(hipStreamSynchronize(c->stream));
     return NBH_OK;
 }
@@ -1275,7 +1291,8 @@ int nbh_ens_autotune(nbh_ensemble* e, int* tiles, int* jsplits, double* ms_out)
         double mn = 1e300;
         for (int rep = 0; rep < 3; ++rep) {
             if (hipEventRecord(a, c->stream) != hipSuccess) return fail(NBH_ERR_HIP, "event record failed");
-            int r = enqueue_ens_force(e, current_state(e), nbh::forces_of(e->f1.d(), e->total), true, false);
+            // into the force buffer not holding the current forces
+            int r = enqueue_ens_force(e, current_state(e), forces_buf(e, 1 - e->parity), true, false);
             if (r) return r;
             if (hipEventRecord(b, c->stream) != hipSuccess) return fail(NBH_ERR_HIP, "event record failed");
             if (hipEventSynchronize(b) != hipSuccess) return fail(NBH_ERR_HIP, "sync failed");
