@@ -434,3 +434,33 @@ def test_zero_softening_steps(nb, oracle, precision):
             assert np.abs(st[:, o:o + n] - ref["state"]).max() <= 1e-12
             o += n
     _banner(f"{precision} zero-softening ensemble steps on both kernels")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_random_tilings_and_step_plans(nb, precision):
+    """Seeded random ensembles (1..40 systems of 1..2000 bodies), random
+    bucket tilings (LDS tiles 64/128/256 or the scalar-load kernel, one
+    split) and random plans that split the steps between eager calls and
+    graph replays of either length parity: every run equals the same number
+    of eager steps on LDS tiles of 256, states and forces bit for bit."""
+    rng = np.random.default_rng(1234 if precision == "fp64" else 4321)
+    for case in range(6):
+        nsys = int(rng.integers(1, 41))
+        counts = rng.integers(1, 2001, nsys).astype(np.int32)
+        eps2 = (1.0 / (4.0 * counts.astype(float) ** (1 / 3))) ** 2
+        state = nb.ensemble_state(counts, 100 + case)
+        tiles = [int(t) for t in rng.choice([0, 64, 128, 256], 4)]
+        plan = [(int(rng.integers(1, 20)), bool(rng.integers(0, 2))) for _ in range(int(rng.integers(1, 5)))]
+        total = sum(n for n, _ in plan)
+        print(f"[gpu-ensemble] case {case}: {nsys} systems, tiles {tiles}, plan {plan}", flush=True)
+        out = {}
+        for name, tl, pl in (("ref", [256] * 4, [(total, False)]), ("run", tiles, plan)):
+            with nb.Backend(precision) as b, nb.Ensemble(b, counts, eps2) as e:
+                e.set_tuning(tl, [1] * 4)
+                e.upload(state)
+                e.set_dt(1.0 / 256)
+                for n, graph in pl:
+                    e.step(n, graph=graph)
+                out[name] = e.download()
+        assert np.array_equal(out["run"][0], out["ref"][0]), (case, tiles, plan)
+        assert np.array_equal(out["run"][1], out["ref"][1]), (case, tiles, plan)
