@@ -1,6 +1,7 @@
 """GPU parity away from the easy cases, through the C ABI: a system far from
-the origin, massless bodies, and rotated inputs, for the directed and the
-symmetric kernels in both precisions.
+the origin, massless bodies, rotated inputs, a bulk velocity and exact
+power-of-two rescaling, for the directed and the symmetric kernels in both
+precisions.
 
 The far-from-origin case pins that the mixed kernels form x_j - x_i in FP64
 and only then round it to FP32, as the reference's _F32 build does
@@ -202,3 +203,31 @@ def test_bulk_velocity(nb, boosted, precision, pairing, sources):
     else:
         assert np.median(ej) <= 1e-6, f"jerk median {np.median(ej):.2e}"
         assert da.max() <= 1e-6 and np.median(dj) <= 1e-7, (da.max(), np.median(dj))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("pairing,sources", KERNELS)
+def test_power_of_two_scale_covariance(nb, precision, pairing, sources):
+    """Positions scaled by 2^-10, velocities by 2^5 and eps2 by 2^-20: every
+    scaling is exact in floating point, and the pair formula is homogeneous,
+    so accelerations must scale by exactly 2^20 and jerks by 2^35 (x^-2 and
+    x^-3 v).  The directed kernels, whose sums have a fixed order, agree bit
+    for bit (the hardware reciprocal square roots scale exactly by powers of
+    4); the symmetric kernels' FP64 atomic sums may add in another order, so
+    they are checked to 1e-13 relative per body."""
+    s = nb.plummer(N, 37)
+    t = s.copy()
+    t[0:3] *= 2.0 ** -10
+    t[4:7] *= 2.0 ** 5
+    f = force(nb, precision, pairing, sources, s)
+    g = force(nb, precision, pairing, sources, t, eps2=EPS2 * 2.0 ** -20)
+    g[0:3] *= 2.0 ** -20
+    g[3:6] *= 2.0 ** -35
+    ea = body_relerr(g[:3], f[:3])
+    ej = body_relerr(g[3:], f[3:])
+    _banner(f"{precision} {pairing}/{sources}: scale covariance max rel acc {ea.max():.1e}, jerk {ej.max():.1e}, "
+            f"bitwise {np.array_equal(g, f)}")
+    assert ea.max() <= 1e-13 and ej.max() <= 1e-13
+    if pairing == "directed":
+        assert np.array_equal(g, f)
