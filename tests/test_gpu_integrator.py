@@ -241,3 +241,29 @@ def test_step_before_upload_is_state_error(nb):
         with pytest.raises(nb.NBodyError) as e:
             b.step(1)   # no dt yet
         assert e.value.code == nb.ERR_STATE
+
+
+def test_hermite_fourth_order_circular_orbit(nb):
+    """Two bodies of mass 1/2 on a circular orbit of unit separation (period
+    2 pi, speed 1/2 each), unsoftened, one period in n fixed steps on the
+    device: the return error |x(2 pi) - x(0)| of a fourth-order Hermite
+    scheme falls as n^-4, so halving dt divides it by ~16.  A predictor or
+    corrector term dropped or mis-weighted (src/nbody.cpp:152-196) makes the
+    scheme lower order and the ratio 4 or 8; checked between 12 and 20 for
+    n = 64 -> 128 -> 256, errors from ~1e-5 down."""
+    s = np.zeros((7, 2))
+    s[0] = [0.5, -0.5]
+    s[3] = [0.5, 0.5]
+    s[5] = [0.5, -0.5]
+    errs = []
+    for n in (64, 128, 256):
+        with nb.Backend("fp64", 0.0) as b:
+            b.upload(s)
+            b.set_dt(2 * np.pi / n)
+            b.step(n)
+            st, _ = b.download()
+        errs.append(np.abs(st[0:3] - s[0:3]).max())
+    ratios = [errs[k] / errs[k + 1] for k in range(2)]
+    _banner(f"circular orbit, one period: errors {['%.3e' % e for e in errs]}, ratios {['%.1f' % r for r in ratios]}")
+    assert errs[0] < 1e-3
+    assert all(12.0 <= r <= 20.0 for r in ratios), ratios
