@@ -23,10 +23,15 @@
 //   x'  = x_p - 0.15 dt^2 (a - a1) - 7/60 dt^3 j - 1/30 dt^3 j1
 //   v'  = v_p - 0.5 dt (a - a1) - 5/12 dt^2 j - 1/12 dt^2 j1
 // and carries (x', v', a1, j1); the corrector's snap/crackle algebra
-// reduces to those coefficients.  Its adjoint is the O(n) kernels k_adj_post
-// and k_adj_pre around one force VJP.  Trajectories keep checkpoints every
-// `every` steps and recompute each segment forward during the backward sweep.
+// reduces to those coefficients.  Its adjoint is two O(n) halves around one
+// force VJP; the second half of step t and the first of step t-1 run as one
+// kernel (k_adj_pre_post).  Trajectories keep checkpoints every `every`
+// steps and recompute each segment forward during the backward sweep,
+// keeping every step's prediction, the VJP's input.  A forward step is the
+// force at the prediction and one kernel (k_adj_advance) that adds the split
+// sums, corrects, and predicts the next step.
 #include "nbh_ctx.h"
+#include "nbh_hermite.h"
 #include "nbh_pair.h"
 
 #include <algorithm>
@@ -159,10 +164,8 @@ __global__ __launch_bounds__(kOB) void k_vjp_reduce(const double* __restrict__ p
 // writes the cotangents of (a1, j1) into cf = [6][n], keeps xbar_p = xbar',
 // vbar_p = vbar' in cot[0..5], and puts the direct part of (abar, jbar)
 // into cot[6..11].
-__global__ __launch_bounds__(kOB) void k_adj_post(int n, double dt, double* cot, double* cf)
+__device__ __forceinline__ void adj_post_body(int i, int n, double dt, double* cot, double* cf)
 {
-    const int i = blockIdx.x * kOB + threadIdx.x;
-    if (i >= n) return;
     const double ca = -0.15 * dt * dt, cj = -7.0 / 60.0 * dt * dt * dt, cj1 = -1.0 / 30.0 * dt * dt * dt;
     const double da = -0.5 * dt, dj = -5.0 / 12.0 * dt * dt, dj1 = -1.0 / 12.0 * dt * dt;
     for (int c = 0; c < 3; ++c) {
@@ -176,19 +179,18 @@ __global__ __launch_bounds__(kOB) void k_adj_post(int n, double dt, double* cot,
     }
 }
 
-// Step adjoint, second half: adds the force VJP (g = [7][n]: x(3) m v(3)) to
-// (xbar_p, vbar_p), accumulates mbar, and pulls the cotangents back through
-// the predictor onto the step's inputs (x, v, a, j).
-__global__ __launch_bounds__(kOB) void k_adj_pre(int n, double dt, double* cot, const double* g, double* mbar)
+// Step adjoint, second half: adds the force VJP g = (x(3) m v(3)) of body i
+// to (xbar_p, vbar_p), accumulates mbar, and pulls the cotangents back
+// through the predictor onto the step's inputs (x, v, a, j).
+__device__ __forceinline__ void adj_pre_body(int i, int n, double dt, double* cot, const double (&g)[7],
+                                             double* mbar)
 {
-    const int i = blockIdx.x * kOB + threadIdx.x;
-    if (i >= n) return;
-    mbar[i] += g[3 * (size_t)n + i];
+    mbar[i] += g[3];
     for (int c = 0; c < 3; ++c) {
         const size_t X = (size_t)c * n + i, V = (size_t)(3 + c) * n + i;
         const size_t A = (size_t)(6 + c) * n + i, J = (size_t)(9 + c) * n + i;
-        const double xp = cot[X] + g[(size_t)c * n + i];
-        const double vp = cot[V] + g[(size_t)(4 + c) * n + i];
+        const double xp = cot[X] + g[c];
+        const double vp = cot[V] + g[4 + c];
         cot[X] = xp;
         cot[V] = fma(xp, dt, vp);
         cot[A] = cot[A] + xp * (0.5 * dt * dt) + vp * dt;
@@ -196,12 +198,126 @@ __global__ __launch_bounds__(kOB) void k_adj_pre(int n, double dt, double* cot, 
     }
 }
 
+__global__ __launch_bounds__(kOB) void k_adj_post(int n, double dt, double* cot, double* cf)
+{
+    const int i = blockIdx.x * kOB + threadIdx.x;
+    if (i >= n) return;
+    adj_post_body(i, n, dt, cot, cf);
+}
+
+// Bodies per workgroup of the two fused step kernels below.  Their split
+// sums take one thread per body and component (a body's six or seven sums
+// done by one thread would run one after another, ~3x slower at n = 4096),
+// combined through LDS for the body's own arithmetic, done by the first wave.
+constexpr int kAB = 64;
+
+// The second half of step t's adjoint, then (post != 0) the first half of
+// step t-1's, for kAB bodies.  g is the force VJP of step t, summed here
+// over its split planes (split s, component c at (s*7 + c)*n) in split
+// order, as k_vjp_reduce does.
+__global__ __launch_bounds__(7 * kAB) void k_adj_pre_post(int n, double dt, double* cot,
+                                                          const double* __restrict__ part, int splits, double* mbar,
+                                                          double* cf, int post)
+{
+    __shared__ double gs[7][kAB];
+    const int b = threadIdx.x % kAB, c = threadIdx.x / kAB;
+    const int i = blockIdx.x * kAB + b;
+    if (i < n) {
+        double v = 0.0;
+#pragma unroll 8
+        for (int s = 0; s < splits; ++s) v += part[((size_t)s * 7 + c) * n + i];
+        gs[c][b] = v;
+    }
+    __syncthreads();
+    if (c != 0 || i >= n) return;
+    double g[7];
+    for (int k = 0; k < 7; ++k) g[k] = gs[k][b];
+    adj_pre_body(i, n, dt, cot, g, mbar);
+    if (post) adj_post_body(i, n, dt, cot, cf);
+}
+
+// The end of one forward step for kAB bodies, fixed dt (*dtp): the new
+// forces (a1, j1) summed over the pair kernel's split planes (component c,
+// split k at (c*splits + k)*n) in split order, as k_reduce_splits does (one
+// split: its plane as is, as the unsplit kernel writes it); the corrector on
+// the step's prediction `pin` (x, m, v: 7n) with the start-of-step forces of
+// `in` (state and forces: 13n), as k_correct; the corrected state and
+// (a1, j1) written to `out` (13n); and, when pnext is set, the next step's
+// prediction from them, as k_predict, written to pnext (7n).  Every input of
+// a body is read before any of its outputs is written, so in == out and
+// pin == pnext are allowed.
+__global__ __launch_bounds__(6 * kAB) void k_adj_advance(int n, const double* dtp, const double* __restrict__ part,
+                                                         int splits, const double* in, const double* pin,
+                                                         double* out, double* pnext)
+{
+#pragma clang fp contract(off)
+    __shared__ double fs[6][kAB];
+    const int b = threadIdx.x % kAB, c = threadIdx.x / kAB;
+    const int i = blockIdx.x * kAB + b;
+    if (i < n) {
+        const double* p = part + (size_t)c * splits * n + i;
+        double v;
+        if (splits == 1) {
+            v = p[0];
+        } else {
+            v = 0.0;
+#pragma unroll 8
+            for (int s = 0; s < splits; ++s) v += p[(size_t)s * n];
+        }
+        fs[c][b] = v;
+    }
+    __syncthreads();
+    if (c != 0 || i >= n) return;
+    const double dt = *dtp;
+    const double invdt2 = 1.0 / (dt * dt);
+    const double invdt3 = 1.0 / (dt * dt * dt);
+    double f1[6], a0[3], j0[3], x[3], u[3];
+    for (int k = 0; k < 6; ++k) f1[k] = fs[k][b];
+    for (int k = 0; k < 3; ++k) {
+        a0[k] = in[(size_t)(7 + k) * n + i];
+        j0[k] = in[(size_t)(10 + k) * n + i];
+        x[k] = pin[(size_t)k * n + i];
+        u[k] = pin[(size_t)(4 + k) * n + i];
+    }
+    const double m = in[(size_t)3 * n + i];
+    for (int k = 0; k < 3; ++k) {
+        double a2, a3;
+        hermite_correct_axis(dt, invdt2, invdt3, a0[k], j0[k], f1[k], f1[3 + k], x[k], u[k], a2, a3);
+    }
+    for (int k = 0; k < 3; ++k) {
+        out[(size_t)k * n + i] = x[k];
+        out[(size_t)(4 + k) * n + i] = u[k];
+        out[(size_t)(7 + k) * n + i] = f1[k];
+        out[(size_t)(10 + k) * n + i] = f1[3 + k];
+    }
+    out[(size_t)3 * n + i] = m;
+    if (pnext) {
+        for (int k = 0; k < 3; ++k) {
+            hermite_predict_axis(dt, f1[k], f1[3 + k], x[k], u[k]);
+            pnext[(size_t)k * n + i] = x[k];
+            pnext[(size_t)(4 + k) * n + i] = u[k];
+        }
+        pnext[(size_t)3 * n + i] = m;
+    }
+}
+
 inline int oblk(int n) { return (n + kOB - 1) / kOB; }
 
 }  // namespace
 
+// Sources per split and split planes the VJP kernel writes for `splits`
+// requested splits.
+int vjp_chunk(int n, int splits)
+{
+    const int tiles = (n + kVB - 1) / kVB;
+    return (tiles + splits - 1) / splits * kVB;
+}
+int vjp_planes(int n, int splits) { return (n + vjp_chunk(n, splits) - 1) / vjp_chunk(n, splits); }
+
 // Force VJP on device arrays: grad = [7][n] (x(3), m, v(3)) of
 // sum_i alpha_i.a_i + beta_i.jerk_i, cot = [6][n].  accumulate adds to grad.
+// With grad null the split planes are left in `partial` (vjp_planes of
+// them, split s and component c at (s*7 + c)*n) for the caller to add.
 hipError_t launch_force_vjp(const State& s, int n, double eps2, const double* cot, double* grad,
                             int accumulate, double* partial, int splits, hipStream_t st)
 {
@@ -210,10 +326,9 @@ hipError_t launch_force_vjp(const State& s, int n, double eps2, const double* co
     a.cot = cot;
     a.out = partial;
     a.n = n;
+    a.jchunk = vjp_chunk(n, splits);
+    const int S = vjp_planes(n, splits);
     const int tiles = (n + kVB - 1) / kVB;
-    const int per = (tiles + splits - 1) / splits;
-    a.jchunk = per * kVB;
-    const int S = (n + a.jchunk - 1) / a.jchunk;
     a.eps2 = eps2;
     const dim3 grid(tiles, S);
     // the self pair is dropped explicitly when eps2 is small enough for its
@@ -224,7 +339,7 @@ hipError_t launch_force_vjp(const State& s, int n, double eps2, const double* co
     else
         k_force_vjp<false><<<grid, kVB, 0, st>>>(a);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || !grad) return e;
     k_vjp_reduce<<<dim3(oblk(n), 7), kOB, 0, st>>>(partial, n, S, grad, accumulate);
     return hipGetLastError();
 }
@@ -260,10 +375,8 @@ struct nbh_adjoint {
     double dt = 0.0;
     bool have_forward = false;
     DevBuf ckpt;     // [ncp][13][n]: state(7) forces(6)
-    DevBuf seg;      // [every][13][n]
     DevBuf cur;      // [13][n]
-    DevBuf f1;       // [6][n]
-    DevBuf pred;     // [7][n]
+    DevBuf pred;     // [every][7][n]: the predictions of a segment's steps
     DevBuf cot;      // [12][n]
     DevBuf cf;       // [6][n]
     DevBuf g;        // [7][n]
@@ -276,39 +389,66 @@ namespace {
 
 int splits_of(const nbh_adjoint* a) { return nbh::vjp_splits(a->n, a->ctx->cus); }
 
-// One FP64 force evaluation of the 7n state at `st` into `f` (6n).
-int adj_force(nbh_adjoint* a, double* st, double* f)
+// Launch shape of the trajectory's FP64 force evaluations: LDS tiles of 64,
+// one target per lane, j-split about 8 workgroups per CU.
+nbh::Shape adj_shape(const nbh_adjoint* a)
 {
-    nbh_ctx* c = a->ctx;
     const int n = a->n;
     nbh::Shape sh{};
     sh.block = 64;
     sh.ipl = 1;
     const int tiles = (n + 63) / 64;
-    const int split = nbh::adj_force_splits(n, c->cus);
+    const int split = nbh::adj_force_splits(n, a->ctx->cus);
     const int per = (tiles + split - 1) / split;
     sh.jchunk = per * 64;
     sh.jsplit = (n + sh.jchunk - 1) / sh.jchunk;
-    hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(st, n), n, c->cfg.eps2, sh, nbh::forces_of(f, n),
-                                     a->partial.d(), nullptr, nullptr, c->stream, nullptr, nullptr);  // LDS tiles
+    return sh;
+}
+
+// One FP64 force evaluation of the 7n state at `st` into `f` (6n).
+int adj_force(nbh_adjoint* a, double* st, double* f)
+{
+    nbh_ctx* c = a->ctx;
+    const int n = a->n;
+    hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(st, n), n, c->cfg.eps2, adj_shape(a),
+                                     nbh::forces_of(f, n), a->partial.d(), nullptr, nullptr, c->stream, nullptr,
+                                     nullptr);  // LDS tiles
     if (e != hipSuccess) return fail(NBH_ERR_HIP, "adjoint force launch failed: %s", hipGetErrorString(e));
     return NBH_OK;
 }
 
-// One forward Hermite step of cur (state + forces) in place.
-int adj_step(nbh_adjoint* a)
+// The prediction of the step that starts at `in` (state and forces, 13n),
+// into p (7n): a copy, then the predictor in place.
+int adj_predict(nbh_adjoint* a, const double* in, double* p)
 {
     nbh_ctx* c = a->ctx;
     const int n = a->n;
-    double* b = a->cur.d();
-    const nbh::Forces f0 = nbh::forces_of(b + 7 * (size_t)n, n);
-    const nbh::Forces f1 = nbh::forces_of(a->f1.d(), n);
-    NBH_HIP(nbh::launch_predict(n, a->dtbuf.d(), b, b + n, b + 2 * (size_t)n, b + 4 * (size_t)n, b + 5 * (size_t)n,
-                                b + 6 * (size_t)n, f0, c->stream));
-    int rc = adj_force(a, b, a->f1.d());
-    if (rc) return rc;
-    NBH_HIP(nbh::launch_correct(n, a->dtbuf.d(), b, b + n, b + 2 * (size_t)n, b + 4 * (size_t)n, b + 5 * (size_t)n,
-                                b + 6 * (size_t)n, f0, f1, nullptr, 0.0, c->stream));
+    NBH_HIP(hipMemcpyAsync(p, in, sizeof(double) * 7 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+    const nbh::Forces f0 = nbh::forces_of(const_cast<double*>(in) + 7 * (size_t)n, n);
+    NBH_HIP(nbh::launch_predict(n, a->dtbuf.d(), p, p + n, p + 2 * (size_t)n, p + 4 * (size_t)n, p + 5 * (size_t)n,
+                                p + 6 * (size_t)n, f0, c->stream));
+    return NBH_OK;
+}
+
+// One forward Hermite step from `in` (13n) with its prediction `pin` (7n):
+// the force at the prediction into the split planes, then k_adj_advance,
+// which sums them, corrects into `out` (13n) and, with pnext set, predicts
+// the next step into pnext (7n).  in == out and pin == pnext are allowed.
+// The states and forces are bitwise those of the step sequence k_predict,
+// force, k_reduce_splits, k_correct.
+int adj_step(nbh_adjoint* a, const double* in, double* pin, double* out, double* pnext)
+{
+    nbh_ctx* c = a->ctx;
+    const int n = a->n;
+    const nbh::Shape sh = adj_shape(a);
+    // one split: the kernel writes the six planes directly, at c*n
+    hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(pin, n), n, c->cfg.eps2, sh,
+                                     nbh::forces_of(a->partial.d(), n), a->partial.d(), nullptr, nullptr, c->stream,
+                                     nullptr, nullptr, 0, false);
+    if (e != hipSuccess) return fail(NBH_ERR_HIP, "adjoint force launch failed: %s", hipGetErrorString(e));
+    nbh::k_adj_advance<<<(n + nbh::kAB - 1) / nbh::kAB, 6 * nbh::kAB, 0, c->stream>>>(
+        n, a->dtbuf.d(), a->partial.d(), sh.jsplit, in, pin, out, pnext);
+    NBH_HIP(hipGetLastError());
     return NBH_OK;
 }
 
@@ -361,8 +501,7 @@ int nbh_adj_create(nbh_ctx* c, int n, int every, nbh_adjoint** out)
     // partial sums of both the force j-split (6 planes per split) and the VJP
     // j-split (7 planes per split)
     const int planes = std::max(6 * nbh::adj_force_splits(n, c->cus), 7 * splits_of(a));
-    if (a->seg.reserve(sz(n, 13 * every)) != hipSuccess || a->cur.reserve(sz(n, 13)) != hipSuccess ||
-        a->f1.reserve(sz(n, 6)) != hipSuccess || a->pred.reserve(sz(n, 7)) != hipSuccess ||
+    if (a->cur.reserve(sz(n, 13)) != hipSuccess || a->pred.reserve(sz(n, 7 * every)) != hipSuccess ||
         a->cot.reserve(sz(n, 12)) != hipSuccess || a->cf.reserve(sz(n, 6)) != hipSuccess ||
         a->g.reserve(sz(n, 7)) != hipSuccess || a->mbar.reserve(sz(n, 1)) != hipSuccess ||
         a->partial.reserve(sz(n, planes)) != hipSuccess || a->dtbuf.reserve(sizeof(double)) != hipSuccess) {
@@ -378,8 +517,7 @@ int nbh_adj_destroy(nbh_adjoint* a)
     if (!a) return NBH_OK;
     (void)hipSetDevice(a->ctx->cfg.device);
     (void)hipStreamSynchronize(a->ctx->stream);
-    for (DevBuf* b : {&a->ckpt, &a->seg, &a->cur, &a->f1, &a->pred, &a->cot, &a->cf, &a->g, &a->mbar, &a->partial,
-                      &a->dtbuf})
+    for (DevBuf* b : {&a->ckpt, &a->cur, &a->pred, &a->cot, &a->cf, &a->g, &a->mbar, &a->partial, &a->dtbuf})
         b->release();
     delete a;
     return NBH_OK;
@@ -405,12 +543,14 @@ int nbh_adj_forward(nbh_adjoint* a, const double* state0, double dt, int nsteps,
     NBH_HIP(hipMemcpyAsync(cur, state0, sz(n, 7), hipMemcpyHostToDevice, c->stream));
     int rc = adj_force(a, cur, cur + 7 * (size_t)n);
     if (rc) return rc;
+    double* p = a->pred.d();
+    if (nsteps > 0 && (rc = adj_predict(a, cur, p))) return rc;
     for (int t = 0; t <= nsteps; ++t) {
         if (t % a->every == 0)
             NBH_HIP(hipMemcpyAsync(a->ckpt.d() + (size_t)(t / a->every) * 13 * n, cur, sz(n, 13),
                                    hipMemcpyDeviceToDevice, c->stream));
         if (t == nsteps) break;
-        if ((rc = adj_step(a))) return rc;
+        if ((rc = adj_step(a, cur, p, cur, t + 1 < nsteps ? p : nullptr))) return rc;
     }
     if (state_final) NBH_HIP(hipMemcpyAsync(state_final, cur, sz(n, 7), hipMemcpyDeviceToHost, c->stream));
     NBH_HIP(hipStreamSynchronize(c->stream));
@@ -437,33 +577,31 @@ int nbh_adj_backward(nbh_adjoint* a, const double* cot_final, double* grad0)
     NBH_HIP(hipMemcpyAsync(a->mbar.p, cot_final + 3 * (size_t)n, sz(n, 1), hipMemcpyHostToDevice, c->stream));
     const int nseg = (a->nsteps + a->every - 1) / a->every;
     const int ob = nbh::oblk(n);
+    const int planes = nbh::vjp_planes(n, splits);
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         const int s0 = sgi * a->every;
         const int s1 = std::min(a->nsteps, s0 + a->every);
-        // recompute the segment from its checkpoint, keeping every step's input
-        NBH_HIP(hipMemcpyAsync(a->cur.p, a->ckpt.d() + (size_t)sgi * 13 * n, sz(n, 13), hipMemcpyDeviceToDevice,
+        // recompute the segment from its checkpoint, keeping every step's
+        // prediction (pred slot t - s0)
+        double* cur = a->cur.d();
+        auto pred_of = [&](int t) { return a->pred.d() + (size_t)(t - s0) * 7 * n; };
+        NBH_HIP(hipMemcpyAsync(cur, a->ckpt.d() + (size_t)sgi * 13 * n, sz(n, 13), hipMemcpyDeviceToDevice,
                                c->stream));
-        for (int t = s0; t < s1; ++t) {
-            NBH_HIP(hipMemcpyAsync(a->seg.d() + (size_t)(t - s0) * 13 * n, a->cur.p, sz(n, 13),
-                                   hipMemcpyDeviceToDevice, c->stream));
-            if (t + 1 < s1) {
-                int rc = adj_step(a);
-                if (rc) return rc;
-            }
-        }
+        int rc = adj_predict(a, cur, pred_of(s0));
+        if (rc) return rc;
+        for (int t = s0; t + 1 < s1; ++t)
+            if ((rc = adj_step(a, cur, pred_of(t), cur, pred_of(t + 1)))) return rc;
         for (int t = s1 - 1; t >= s0; --t) {
-            const double* in = a->seg.d() + (size_t)(t - s0) * 13 * n;
-            nbh::k_adj_post<<<ob, nbh::kOB, 0, c->stream>>>(n, dt, cot, a->cf.d());
-            NBH_HIP(hipGetLastError());
-            // predicted state of step t
-            double* p = a->pred.d();
-            NBH_HIP(hipMemcpyAsync(p, in, sz(n, 7), hipMemcpyDeviceToDevice, c->stream));
-            const nbh::Forces f0 = nbh::forces_of(const_cast<double*>(in) + 7 * (size_t)n, n);
-            NBH_HIP(nbh::launch_predict(n, a->dtbuf.d(), p, p + n, p + 2 * (size_t)n, p + 4 * (size_t)n,
-                                        p + 5 * (size_t)n, p + 6 * (size_t)n, f0, c->stream));
-            NBH_HIP(nbh::launch_force_vjp(nbh::state_of(p, n), n, c->cfg.eps2, a->cf.d(), a->g.d(), 0,
+            // the first half of step t's adjoint: the last step's here, the
+            // others' fused into the second half of step t + 1's
+            if (t == a->nsteps - 1) {
+                nbh::k_adj_post<<<ob, nbh::kOB, 0, c->stream>>>(n, dt, cot, a->cf.d());
+                NBH_HIP(hipGetLastError());
+            }
+            NBH_HIP(nbh::launch_force_vjp(nbh::state_of(pred_of(t), n), n, c->cfg.eps2, a->cf.d(), nullptr, 0,
                                           a->partial.d(), splits, c->stream));
-            nbh::k_adj_pre<<<ob, nbh::kOB, 0, c->stream>>>(n, dt, cot, a->g.d(), a->mbar.d());
+            nbh::k_adj_pre_post<<<(n + nbh::kAB - 1) / nbh::kAB, 7 * nbh::kAB, 0, c->stream>>>(
+                n, dt, cot, a->partial.d(), planes, a->mbar.d(), a->cf.d(), t > 0);
             NBH_HIP(hipGetLastError());
         }
     }

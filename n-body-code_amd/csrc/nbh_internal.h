@@ -61,7 +61,9 @@ struct Shape {
 // Pair-kernel launch: writes forces for all n targets from all n sources
 // (from the first nj sources when 0 < nj < n: the autotuner's short trial
 // launches).  When shape.jsplit > 1, `partial` must hold 6*jsplit*n doubles
-// and the split-reduction kernel is launched after the pair kernel.  ev0/ev1
+// and the split-reduction kernel is launched after the pair kernel, unless
+// reduce_splits is false: then the partial planes (component c, split k at
+// (c*jsplit + k)*n) are left for the caller to add in split order.  ev0/ev1
 // (may be null) are recorded around the pair kernel only.
 // rec: with shape.scalar, a device buffer of source_record_bytes(precision,
 // n) bytes for the packed source records (unused, may be null, otherwise).
@@ -76,7 +78,7 @@ hipError_t pack_source_records(int precision, const State& s, int n, const doubl
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out,
                         double* partial, void* rec, double* vwork, hipStream_t st, hipEvent_t ev0,
-                        hipEvent_t ev1, int nj = 0);
+                        hipEvent_t ev1, int nj = 0, bool reduce_splits = true);
 
 // Directed pair kernel restricted to the diagonal blocks: target i gets the
 // sources of its own block [B*(i/B), B*(i/B)+B), self-pair excluded; the

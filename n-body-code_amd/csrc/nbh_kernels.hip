@@ -733,7 +733,8 @@ hipError_t launch_vref(const State& s, int n, double* work, hipStream_t st)
 
 hipError_t launch_force(int precision, const State& s, int n, double eps2,
                         const Shape& shape, const Forces& out, double* partial,
-                        void* rec, double* vwork, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int nj)
+                        void* rec, double* vwork, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int nj,
+                        bool reduce_splits)
 {
     if (precision != 0 && !vwork) return hipErrorInvalidValue;
     PairArgs a;
@@ -774,7 +775,7 @@ hipError_t launch_force(int precision, const State& s, int n, double eps2,
     }
     if (e != hipSuccess) return e;
     if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
-    if (shape.jsplit > 1) {
+    if (shape.jsplit > 1 && reduce_splits) {
         k_reduce_splits<<<dim3(oblocks(n), 6), kOBlock, 0, st>>>(partial, n, shape.jsplit, out);
         e = hipGetLastError();
     }
