@@ -59,13 +59,17 @@ def test_pair_force_gradcheck(nb, be):
     be.configure(eps2=1.0 / 4096)
 
 
-def test_trajectory_adjoint_vs_torch_reference(nb, be):
-    """N=128, 32 steps, checkpoints every 8: the adjoint gradient of
-    sum |x(T) - t|^2 + sum w.v(T) against torch autograd of the reference."""
+@pytest.mark.parametrize("n", [40, 128])
+def test_trajectory_adjoint_vs_torch_reference(nb, be, n):
+    """32 steps, checkpoints every 8: the final state of the forward run and
+    the adjoint gradient of sum |x(T) - t|^2 + sum w.v(T) against the torch
+    reference and its autograd.  n = 40 fits one source tile, so the force
+    is evaluated in one split and the step kernel takes its planes as they
+    are; n = 128 runs two splits."""
     import torch
     import hermite_torch as ht
     from nbody_amd.autograd import trajectory
-    n, dt, nsteps, eps2 = 128, 1.0 / 512, 32, 1.0 / 4096
+    dt, nsteps, eps2 = 1.0 / 512, 32, 1.0 / 4096
     s = unequal(nb, n, 17)
     rng = np.random.default_rng(9)
     tgt = torch.from_numpy(s[0:3] + rng.normal(scale=0.01, size=(3, n)))
@@ -77,12 +81,17 @@ def test_trajectory_adjoint_vs_torch_reference(nb, be):
 
     leaves = [torch.from_numpy(s[0:3].copy()).requires_grad_(), torch.from_numpy(s[4:7].copy()).requires_grad_(),
               torch.from_numpy(s[3].copy()).requires_grad_()]
-    loss(*trajectory(*leaves, be, dt, nsteps, 8)).backward()
+    xT, vT = trajectory(*leaves, be, dt, nsteps, 8)
+    loss(xT, vT).backward()
     mine = [l.grad.numpy().copy() for l in leaves]
     ref_leaves = [l.detach().clone().requires_grad_() for l in leaves]
-    loss(*ht.trajectory(*ref_leaves, dt, nsteps, eps2)).backward()
+    rxT, rvT = ht.trajectory(*ref_leaves, dt, nsteps, eps2)
+    loss(rxT, rvT).backward()
+    ex, ev = rel(xT.detach().numpy(), rxT.detach().numpy()), rel(vT.detach().numpy(), rvT.detach().numpy())
     errs = [rel(a, b.grad.numpy()) for a, b in zip(mine, ref_leaves)]
-    _banner(f"trajectory adjoint vs torch reference: rel err x {errs[0]:.1e} v {errs[1]:.1e} m {errs[2]:.1e}")
+    _banner(f"n={n} forward vs torch reference: rel err x {ex:.1e} v {ev:.1e}; adjoint: rel err x {errs[0]:.1e} "
+            f"v {errs[1]:.1e} m {errs[2]:.1e}")
+    assert max(ex, ev) < 1e-13
     assert max(errs) < 1e-10
 
 
