@@ -307,19 +307,19 @@ inline int oblk(int n) { return (n + kOB - 1) / kOB; }
 
 // Sources per split and split planes the VJP kernel writes for `splits`
 // requested splits.
-int vjp_chunk(int n, int splits)
+static int vjp_chunk(int n, int splits)
 {
     const int tiles = (n + kVB - 1) / kVB;
     return (tiles + splits - 1) / splits * kVB;
 }
-int vjp_planes(int n, int splits) { return (n + vjp_chunk(n, splits) - 1) / vjp_chunk(n, splits); }
+static int vjp_planes(int n, int splits) { return (n + vjp_chunk(n, splits) - 1) / vjp_chunk(n, splits); }
 
 // Force VJP on device arrays: grad = [7][n] (x(3), m, v(3)) of
 // sum_i alpha_i.a_i + beta_i.jerk_i, cot = [6][n].  accumulate adds to grad.
 // With grad null the split planes are left in `partial` (vjp_planes of
 // them, split s and component c at (s*7 + c)*n) for the caller to add.
-hipError_t launch_force_vjp(const State& s, int n, double eps2, const double* cot, double* grad,
-                            int accumulate, double* partial, int splits, hipStream_t st)
+static hipError_t launch_force_vjp(const State& s, int n, double eps2, const double* cot, double* grad,
+                                   int accumulate, double* partial, int splits, hipStream_t st)
 {
     VjpArgs a;
     a.s = s;
@@ -346,7 +346,7 @@ hipError_t launch_force_vjp(const State& s, int n, double eps2, const double* co
 
 // j-split of the FP64 force evaluations inside the trajectory (64-lane
 // workgroups, about 8 per CU; 16 per CU measured slower at n = 4096).
-int adj_force_splits(int n, int cus)
+static int adj_force_splits(int n, int cus)
 {
     const int tiles = (n + 63) / 64;
     return std::max(1, std::min(tiles, (8 * (cus > 0 ? cus : 256) + tiles - 1) / tiles));
@@ -354,7 +354,7 @@ int adj_force_splits(int n, int cus)
 
 // j-split of the force VJP: one-wave workgroups, about 8 per CU (at n = 4096
 // 6% faster than 4 per CU; 16 per CU no faster).
-int vjp_splits(int n, int cus)
+static int vjp_splits(int n, int cus)
 {
     const int tiles = (n + kVB - 1) / kVB;
     const int want = 8 * (cus > 0 ? cus : 256);
