@@ -460,11 +460,14 @@ def main():
     err = body_relerr(fmx[:3], f64[:3])
     med, mx = float(np.median(err)), float(err.max())
     errd = body_relerr(fmd[:3], f64[:3])
+    # the jerk (adot) beside it, per body against the same FP64 forces
+    errj, errdj = body_relerr(fmx[3:], f64[3:]), body_relerr(fmd[3:], f64[3:])
     # symmetric vs directed FP64, per body, acceleration and jerk
     err64s = np.maximum(body_relerr(f64s[:3], f64[:3]), body_relerr(f64s[3:], f64[3:]))
     log(f"mixed vs fp64 acc rel err: median {med:.2e}, max {mx:.2e} "
-        f"(directed: median {np.median(errd):.2e}, max {errd.max():.2e}); "
-        f"fp64 symmetric vs directed max {err64s.max():.2e}")
+        f"(directed: median {np.median(errd):.2e}, max {errd.max():.2e}); jerk rel err: median "
+        f"{np.median(errj):.2e}, max {errj.max():.2e} (directed: median {np.median(errdj):.2e}, max "
+        f"{errdj.max():.2e}); fp64 symmetric vs directed max {err64s.max():.2e}")
 
     ens = None if args.no_ensemble else run_ensemble(nb, log)
     adj = None if args.no_ensemble else run_adjoint(nb, log)
@@ -498,6 +501,8 @@ def main():
         "valid": med <= 1e-6,
         "mixed_median_rel_acc_err": med,
         "mixed_max_rel_acc_err": mx,
+        "mixed_median_rel_jerk_err": float(np.median(errj)),
+        "mixed_max_rel_jerk_err": float(errj.max()),
         "fp64_pairs_per_s": r64["pairs_per_s"],
         "mixed_over_fp64": {"symmetric_over_symmetric": ratio_ss,
                             "directed_over_directed": ratio_dd,
@@ -515,6 +520,7 @@ def main():
                            "launch_shape": r64s["shape"], "roofline": roofline("fp64", r64s, traffic, lease)},
         "mixed_directed": {"pairs_per_s": rmd["pairs_per_s"], "over_fp64": rmd["pairs_per_s"] / r64["pairs_per_s"],
                            "median_rel_acc_err": float(np.median(errd)), "max_rel_acc_err": float(errd.max()),
+                           "median_rel_jerk_err": float(np.median(errdj)), "max_rel_jerk_err": float(errdj.max()),
                            "energy_drift": rmd["energy_drift"], "launch_shape": rmd["shape"],
                            "roofline": roofline("mixed", rmd, traffic, lease)},
         "steps_per_s": {"fp64": r64["steps_per_s"], "fp64_symmetric": r64s["steps_per_s"],
