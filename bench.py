@@ -328,8 +328,8 @@ def run_adjoint(nb, log):
 
 def host_info():
     """CPU model and counts of this host, and the CPU share this process may
-    use (OMP_NUM_THREADS when set -- the GPU box allots each GPU's jobs a
-    share of its cores --, else the CPUs it may run on)."""
+    use (OMP_NUM_THREADS when set -- a shared host may allot each GPU's jobs
+    a share of its cores --, else the CPUs it may run on)."""
     model = None
     try:
         for line in open("/proc/cpuinfo"):

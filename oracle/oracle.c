@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU restatement of the reference's Hermite-4 force path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the
+ * TEST INFRASTRUCTURE ONLY.  The parity tests compare against it: only tests/, the
  * smoke() entry and bench.py's cpu_baseline leg may load it.  The product
  * (n-body-code_amd/) never links, loads or falls back to it.
  *

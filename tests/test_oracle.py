@@ -1,4 +1,4 @@
-"""Pins the oracle (oracle/oracle.c) before it is trusted as the checker.
+"""Pins the oracle (oracle/oracle.c) before the parity tests trust it.
 
 Each function is checked against the reference's own outputs (golden
 fixtures written from the compiled reference, and the compiled reference
