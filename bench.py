@@ -506,7 +506,7 @@ def main():
         "dtype": "f64/f32 mixed",
         "data": "synthetic (Plummer sphere, generated in-process)",
         "config": {"workload": "Hermite-4 shared step, N=1048576 Plummer seed 3, eps=1e-3, dt=1/2048",
-                   "n_bodies": N, "global_batch": N, "seq_len": None, "parallelism": "none (1 GPU)"},
+                   "n_bodies": N, "steps_timed": args.steps, "parallelism": "none (1 GPU)"},
         "valid": med <= 1e-6,
         "mixed_median_rel_acc_err": med,
         "mixed_max_rel_acc_err": mx,
