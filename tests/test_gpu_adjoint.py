@@ -153,3 +153,29 @@ def test_config5_adjoint_vs_central_differences(nb, be):
             worst = max(worst, abs(grad[r, c] - fd) / scale)
     _banner(f"config5 adjoint vs central differences (32 coords): worst rel err {worst:.2e}")
     assert worst < 1e-4
+
+
+@pytest.mark.parametrize("n", [40, 1000])
+def test_forward_steps_bitwise_those_of_the_resident_integrator(nb, n):
+    """The adjoint's forward step (force at the prediction, then one kernel
+    that adds the split sums, corrects and predicts the next step) against
+    the resident integrator's k_predict, force, k_reduce_splits, k_correct,
+    with the force launched in the same shape (LDS tiles of 64, one target
+    per lane, the adjoint's j-split of about 8 workgroups per CU): the states
+    after 24 steps are bitwise equal.  n = 40 is one split (the planes as the
+    unsplit kernel writes them), n = 1000 sixteen."""
+    dt, nsteps, eps2 = 1.0 / 512, 24, 1.0 / 4096
+    cus = nb.device_info()["compute_units"]
+    tiles = (n + 63) // 64
+    splits = max(1, min(tiles, (8 * cus + tiles - 1) // tiles))
+    s = unequal(nb, n, 31)
+    with nb.Backend("fp64", eps2, block=64, jsplit=splits, ipl=1, sources="lds") as b:
+        b.upload(s)
+        b.set_dt(dt)
+        b.step(nsteps)
+        resident, _ = b.download()
+        with nb.Adjoint(b, n, 8) as adj:
+            fin = adj.forward(s, dt, nsteps)
+    diff = np.abs(fin - resident).max()
+    _banner(f"n={n}, {splits} splits: adjoint forward vs resident steps, max |difference| {diff:.1e}")
+    assert np.array_equal(fin, resident)
