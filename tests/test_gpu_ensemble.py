@@ -464,3 +464,35 @@ def test_random_tilings_and_step_plans(nb, precision):
                 out[name] = e.download()
         assert np.array_equal(out["run"][0], out["ref"][0]), (case, tiles, plan)
         assert np.array_equal(out["run"][1], out["ref"][1]), (case, tiles, plan)
+
+
+@pytest.mark.parametrize("tile", [64, 0])
+def test_systems_bitwise_those_of_single_system_runs(nb, small, tile):
+    """FP64, every bucket unsplit (LDS tiles of 64, or the scalar-load
+    kernel): each target sums its sources in order from the system's first
+    body, as the single-system LDS kernel does with one split, and the step
+    uses the same predictor and corrector arithmetic.  So every system's
+    initial forces, and its state and forces after 6 steps, equal those of
+    the same system run alone, bit for bit."""
+    counts, eps2, state = small
+    dt, nsteps = 1.0 / 256, 6
+    with nb.Backend("fp64") as b, nb.Ensemble(b, counts, eps2) as e:
+        e.set_tuning([tile] * 4, [1] * 4)
+        e.upload(state)
+        _, f0 = e.download()
+        e.set_dt(dt)
+        e.step(nsteps)
+        sT, fT = e.download()
+    off = np.concatenate([[0], np.cumsum(counts)])
+    for k in range(len(counts)):
+        a, z = off[k], off[k + 1]
+        with nb.Backend("fp64", eps2[k], block=64, jsplit=1, ipl=1, sources="lds") as b1:
+            b1.upload(state[:, a:z])
+            _, g0 = b1.download()
+            b1.set_dt(dt)
+            b1.step(nsteps)
+            s1, g1 = b1.download()
+        assert np.array_equal(f0[:, a:z], g0), f"system {k} (n={counts[k]}): initial forces"
+        assert np.array_equal(sT[:, a:z], s1), f"system {k} (n={counts[k]}): state after {nsteps} steps"
+        assert np.array_equal(fT[:, a:z], g1), f"system {k} (n={counts[k]}): forces after {nsteps} steps"
+    _banner(f"tile {tile}: {len(counts)} systems bitwise equal to single-system runs over {nsteps} steps")
