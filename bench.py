@@ -299,7 +299,8 @@ def run_adjoint(nb, log):
     lognormal(0.1) renormalised to 1, eps=1/64, dt=1/512, 128 FP64 steps,
     checkpoints every 16; loss = sum |x(T) - x_target|^2 with targets from a
     run with re-drawn masses (seed 6).  Times the forward run and the
-    checkpointed backward sweep (which recomputes each segment)."""
+    checkpointed backward sweep (which recomputes each segment), after one
+    untimed forward run (the targets) and one untimed sweep."""
     n, dt, nsteps, every = 4096, 1.0 / 512, 128, 16
     s = nb.plummer(n, 5)
     m = s[3] * np.random.Generator(np.random.PCG64(5)).lognormal(0.0, 0.1, n)
@@ -309,6 +310,9 @@ def run_adjoint(nb, log):
     st[3] = mt / mt.sum()
     with nb.Backend("fp64", (1.0 / 64) ** 2) as b, nb.Adjoint(b, n, every) as adj:
         target = adj.forward(st, dt, nsteps)[0:3]
+        # untimed warm-up sweep: a process's first backward sweep pays a
+        # one-time cost of a few ms (the first concurrent use of its two streams)
+        adj.backward(np.zeros((7, n)))
         t0 = time.perf_counter()
         fin = adj.forward(s, dt, nsteps)
         t1 = time.perf_counter()

@@ -27,9 +27,10 @@
 // force VJP; the second half of step t and the first of step t-1 run as one
 // kernel (k_adj_pre_post).  Trajectories keep checkpoints every `every`
 // steps and recompute each segment forward during the backward sweep,
-// keeping every step's prediction, the VJP's input.  A forward step is the
-// force at the prediction and one kernel (k_adj_advance) that adds the split
-// sums, corrects, and predicts the next step.
+// keeping every step's prediction, the VJP's input; the recompute of segment
+// k-1 runs on a second stream beside segment k's step adjoints.  A forward
+// step is the force at the prediction and one kernel (k_adj_advance) that
+// adds the split sums, corrects, and predicts the next step.
 #include "nbh_ctx.h"
 #include "nbh_hermite.h"
 #include "nbh_pair.h"
@@ -374,15 +375,22 @@ struct nbh_adjoint {
     int nsteps = 0;
     double dt = 0.0;
     bool have_forward = false;
-    DevBuf ckpt;     // [ncp][13][n]: state(7) forces(6)
-    DevBuf cur;      // [13][n]
-    DevBuf pred;     // [every][7][n]: the predictions of a segment's steps
-    DevBuf cot;      // [12][n]
-    DevBuf cf;       // [6][n]
-    DevBuf g;        // [7][n]
-    DevBuf mbar;     // [n]
-    DevBuf partial;  // [splits][7][n] (also the force split partials)
-    DevBuf dtbuf;    // [1]
+    DevBuf ckpt;      // [ncp][13][n]: state(7) forces(6)
+    DevBuf cur;       // [13][n]: the recompute's working state
+    DevBuf pred;      // [2][every][7][n]: the predictions of two segments' steps
+    DevBuf cot;       // [12][n]
+    DevBuf cf;        // [6][n]
+    DevBuf g;         // [7][n]
+    DevBuf mbar;      // [n]
+    DevBuf partial;   // [splits][7][n]: the VJP's split planes (and the forward run's force planes)
+    DevBuf rpartial;  // [6][splits][n]: the recompute's force split planes
+    DevBuf dtbuf;     // [1]
+    // The backward sweep recomputes segment k-1 on its own stream while the
+    // context's stream runs segment k's step adjoints (default priority).
+    hipStream_t rstream = nullptr;
+    hipEvent_t ev_start = nullptr;    // context stream: the sweep's inputs are in place
+    hipEvent_t ev_ready[2] = {};      // recompute stream: pred[b] holds its segment
+    hipEvent_t ev_free[2] = {};       // context stream: pred[b] has been used
 };
 
 namespace {
@@ -418,37 +426,58 @@ int adj_force(nbh_adjoint* a, double* st, double* f)
 }
 
 // The prediction of the step that starts at `in` (state and forces, 13n),
-// into p (7n): a copy, then the predictor in place.
-int adj_predict(nbh_adjoint* a, const double* in, double* p)
+// into p (7n), on stream st: a copy, then the predictor in place.
+int adj_predict(nbh_adjoint* a, const double* in, double* p, hipStream_t st)
 {
-    nbh_ctx* c = a->ctx;
     const int n = a->n;
-    NBH_HIP(hipMemcpyAsync(p, in, sizeof(double) * 7 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+    NBH_HIP(hipMemcpyAsync(p, in, sizeof(double) * 7 * (size_t)n, hipMemcpyDeviceToDevice, st));
     const nbh::Forces f0 = nbh::forces_of(const_cast<double*>(in) + 7 * (size_t)n, n);
     NBH_HIP(nbh::launch_predict(n, a->dtbuf.d(), p, p + n, p + 2 * (size_t)n, p + 4 * (size_t)n, p + 5 * (size_t)n,
-                                p + 6 * (size_t)n, f0, c->stream));
+                                p + 6 * (size_t)n, f0, st));
     return NBH_OK;
 }
 
-// One forward Hermite step from `in` (13n) with its prediction `pin` (7n):
-// the force at the prediction into the split planes, then k_adj_advance,
-// which sums them, corrects into `out` (13n) and, with pnext set, predicts
-// the next step into pnext (7n).  in == out and pin == pnext are allowed.
-// The states and forces are bitwise those of the step sequence k_predict,
-// force, k_reduce_splits, k_correct.
-int adj_step(nbh_adjoint* a, const double* in, double* pin, double* out, double* pnext)
+// One forward Hermite step from `in` (13n) with its prediction `pin` (7n),
+// on stream st with the split planes in `part`: the force at the prediction
+// into the planes, then k_adj_advance, which sums them, corrects into `out`
+// (13n) and, with pnext set, predicts the next step into pnext (7n).
+// in == out and pin == pnext are allowed.  The states and forces are
+// bitwise those of the step sequence k_predict, force, k_reduce_splits,
+// k_correct.
+int adj_step(nbh_adjoint* a, const double* in, double* pin, double* out, double* pnext, hipStream_t st,
+             double* part)
 {
     nbh_ctx* c = a->ctx;
     const int n = a->n;
     const nbh::Shape sh = adj_shape(a);
     // one split: the kernel writes the six planes directly, at c*n
-    hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(pin, n), n, c->cfg.eps2, sh,
-                                     nbh::forces_of(a->partial.d(), n), a->partial.d(), nullptr, nullptr, c->stream,
-                                     nullptr, nullptr, 0, false);
+    hipError_t e = nbh::launch_force(NBH_FP64, nbh::state_of(pin, n), n, c->cfg.eps2, sh, nbh::forces_of(part, n),
+                                     part, nullptr, nullptr, st, nullptr, nullptr, 0, false);
     if (e != hipSuccess) return fail(NBH_ERR_HIP, "adjoint force launch failed: %s", hipGetErrorString(e));
-    nbh::k_adj_advance<<<(n + nbh::kAB - 1) / nbh::kAB, 6 * nbh::kAB, 0, c->stream>>>(
-        n, a->dtbuf.d(), a->partial.d(), sh.jsplit, in, pin, out, pnext);
+    nbh::k_adj_advance<<<(n + nbh::kAB - 1) / nbh::kAB, 6 * nbh::kAB, 0, st>>>(n, a->dtbuf.d(), part, sh.jsplit, in,
+                                                                               pin, out, pnext);
     NBH_HIP(hipGetLastError());
+    return NBH_OK;
+}
+
+// Segment sgi (steps s0 .. s1-1) recomputed from its checkpoint on the
+// recompute stream, keeping every step's prediction: step t's in slot
+// t - s0 of `pb` ([every][7][n]).
+int adj_recompute(nbh_adjoint* a, int sgi, double* pb)
+{
+    const int n = a->n;
+    const int s0 = sgi * a->every;
+    const int s1 = std::min(a->nsteps, s0 + a->every);
+    hipStream_t st = a->rstream;
+    double* cur = a->cur.d();
+    NBH_HIP(hipMemcpyAsync(cur, a->ckpt.d() + (size_t)sgi * 13 * n, sizeof(double) * 13 * (size_t)n,
+                           hipMemcpyDeviceToDevice, st));
+    int rc = adj_predict(a, cur, pb, st);
+    if (rc) return rc;
+    for (int t = s0; t + 1 < s1; ++t) {
+        double* p = pb + (size_t)(t - s0) * 7 * n;
+        if ((rc = adj_step(a, cur, p, cur, p + 7 * (size_t)n, st, a->rpartial.d()))) return rc;
+    }
     return NBH_OK;
 }
 
@@ -501,12 +530,20 @@ int nbh_adj_create(nbh_ctx* c, int n, int every, nbh_adjoint** out)
     // partial sums of both the force j-split (6 planes per split) and the VJP
     // j-split (7 planes per split)
     const int planes = std::max(6 * nbh::adj_force_splits(n, c->cus), 7 * splits_of(a));
-    if (a->cur.reserve(sz(n, 13)) != hipSuccess || a->pred.reserve(sz(n, 7 * every)) != hipSuccess ||
+    if (a->cur.reserve(sz(n, 13)) != hipSuccess || a->pred.reserve(sz(n, 14 * every)) != hipSuccess ||
+        a->rpartial.reserve(sz(n, 6 * nbh::adj_force_splits(n, c->cus))) != hipSuccess ||
         a->cot.reserve(sz(n, 12)) != hipSuccess || a->cf.reserve(sz(n, 6)) != hipSuccess ||
         a->g.reserve(sz(n, 7)) != hipSuccess || a->mbar.reserve(sz(n, 1)) != hipSuccess ||
         a->partial.reserve(sz(n, planes)) != hipSuccess || a->dtbuf.reserve(sizeof(double)) != hipSuccess) {
         nbh_adj_destroy(a);
         return fail(NBH_ERR_ALLOC, "adjoint device allocation failed (n=%d, every=%d)", n, every);
+    }
+    hipError_t he = hipStreamCreateWithFlags(&a->rstream, hipStreamNonBlocking);
+    for (hipEvent_t* ev : {&a->ev_start, &a->ev_ready[0], &a->ev_ready[1], &a->ev_free[0], &a->ev_free[1]})
+        if (he == hipSuccess) he = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    if (he != hipSuccess) {
+        nbh_adj_destroy(a);
+        return fail(NBH_ERR_HIP, "adjoint stream/event creation failed: %s", hipGetErrorString(he));
     }
     *out = a;
     return NBH_OK;
@@ -517,7 +554,12 @@ int nbh_adj_destroy(nbh_adjoint* a)
     if (!a) return NBH_OK;
     (void)hipSetDevice(a->ctx->cfg.device);
     (void)hipStreamSynchronize(a->ctx->stream);
-    for (DevBuf* b : {&a->ckpt, &a->cur, &a->pred, &a->cot, &a->cf, &a->g, &a->mbar, &a->partial, &a->dtbuf})
+    if (a->rstream) (void)hipStreamSynchronize(a->rstream);
+    for (hipEvent_t ev : {a->ev_start, a->ev_ready[0], a->ev_ready[1], a->ev_free[0], a->ev_free[1]})
+        if (ev) (void)hipEventDestroy(ev);
+    if (a->rstream) (void)hipStreamDestroy(a->rstream);
+    for (DevBuf* b : {&a->ckpt, &a->cur, &a->pred, &a->cot, &a->cf, &a->g, &a->mbar, &a->partial, &a->rpartial,
+                      &a->dtbuf})
         b->release();
     delete a;
     return NBH_OK;
@@ -544,13 +586,13 @@ int nbh_adj_forward(nbh_adjoint* a, const double* state0, double dt, int nsteps,
     int rc = adj_force(a, cur, cur + 7 * (size_t)n);
     if (rc) return rc;
     double* p = a->pred.d();
-    if (nsteps > 0 && (rc = adj_predict(a, cur, p))) return rc;
+    if (nsteps > 0 && (rc = adj_predict(a, cur, p, c->stream))) return rc;
     for (int t = 0; t <= nsteps; ++t) {
         if (t % a->every == 0)
             NBH_HIP(hipMemcpyAsync(a->ckpt.d() + (size_t)(t / a->every) * 13 * n, cur, sz(n, 13),
                                    hipMemcpyDeviceToDevice, c->stream));
         if (t == nsteps) break;
-        if ((rc = adj_step(a, cur, p, cur, t + 1 < nsteps ? p : nullptr))) return rc;
+        if ((rc = adj_step(a, cur, p, cur, t + 1 < nsteps ? p : nullptr, c->stream, a->partial.d()))) return rc;
     }
     if (state_final) NBH_HIP(hipMemcpyAsync(state_final, cur, sz(n, 7), hipMemcpyDeviceToHost, c->stream));
     NBH_HIP(hipStreamSynchronize(c->stream));
@@ -576,34 +618,49 @@ int nbh_adj_backward(nbh_adjoint* a, const double* cot_final, double* grad0)
                            c->stream));
     NBH_HIP(hipMemcpyAsync(a->mbar.p, cot_final + 3 * (size_t)n, sz(n, 1), hipMemcpyHostToDevice, c->stream));
     const int nseg = (a->nsteps + a->every - 1) / a->every;
-    const int ob = nbh::oblk(n);
     const int planes = nbh::vjp_planes(n, splits);
+    auto pbuf = [&](int sgi) { return a->pred.d() + (size_t)(sgi & 1) * a->every * 7 * n; };
+    // The recompute stream produces segment k-1's predictions while this
+    // stream runs segment k's step adjoints; segments alternate between the
+    // two halves of pred.  ev_ready[b]: half b holds its segment; ev_free[b]:
+    // this stream is done with half b, which the recompute may overwrite.
+    NBH_HIP(hipEventRecord(a->ev_start, c->stream));
+    NBH_HIP(hipStreamWaitEvent(a->rstream, a->ev_start, 0));
+    // on a failure, nothing of the sweep is left running on the recompute stream
+    auto failed = [&](int rc) {
+        (void)hipStreamSynchronize(a->rstream);
+        return rc;
+    };
+    if (nseg > 0) {
+        if (int rc = adj_recompute(a, nseg - 1, pbuf(nseg - 1))) return failed(rc);
+        NBH_HIP(hipEventRecord(a->ev_ready[(nseg - 1) & 1], a->rstream));
+    }
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+        const int b = sgi & 1;
+        if (sgi > 0) {
+            // half (sgi-1)&1 last held segment sgi+1, if that exists
+            if (sgi + 1 < nseg) NBH_HIP(hipStreamWaitEvent(a->rstream, a->ev_free[(sgi - 1) & 1], 0));
+            if (int rc = adj_recompute(a, sgi - 1, pbuf(sgi - 1))) return failed(rc);
+            NBH_HIP(hipEventRecord(a->ev_ready[(sgi - 1) & 1], a->rstream));
+        }
+        NBH_HIP(hipStreamWaitEvent(c->stream, a->ev_ready[b], 0));
         const int s0 = sgi * a->every;
         const int s1 = std::min(a->nsteps, s0 + a->every);
-        // recompute the segment from its checkpoint, keeping every step's
-        // prediction (pred slot t - s0)
-        double* cur = a->cur.d();
-        auto pred_of = [&](int t) { return a->pred.d() + (size_t)(t - s0) * 7 * n; };
-        NBH_HIP(hipMemcpyAsync(cur, a->ckpt.d() + (size_t)sgi * 13 * n, sz(n, 13), hipMemcpyDeviceToDevice,
-                               c->stream));
-        int rc = adj_predict(a, cur, pred_of(s0));
-        if (rc) return rc;
-        for (int t = s0; t + 1 < s1; ++t)
-            if ((rc = adj_step(a, cur, pred_of(t), cur, pred_of(t + 1)))) return rc;
         for (int t = s1 - 1; t >= s0; --t) {
             // the first half of step t's adjoint: the last step's here, the
             // others' fused into the second half of step t + 1's
             if (t == a->nsteps - 1) {
-                nbh::k_adj_post<<<ob, nbh::kOB, 0, c->stream>>>(n, dt, cot, a->cf.d());
+                nbh::k_adj_post<<<nbh::oblk(n), nbh::kOB, 0, c->stream>>>(n, dt, cot, a->cf.d());
                 NBH_HIP(hipGetLastError());
             }
-            NBH_HIP(nbh::launch_force_vjp(nbh::state_of(pred_of(t), n), n, c->cfg.eps2, a->cf.d(), nullptr, 0,
-                                          a->partial.d(), splits, c->stream));
+            const double* p = pbuf(sgi) + (size_t)(t - s0) * 7 * n;
+            NBH_HIP(nbh::launch_force_vjp(nbh::state_of(const_cast<double*>(p), n), n, c->cfg.eps2, a->cf.d(),
+                                          nullptr, 0, a->partial.d(), splits, c->stream));
             nbh::k_adj_pre_post<<<(n + nbh::kAB - 1) / nbh::kAB, 7 * nbh::kAB, 0, c->stream>>>(
                 n, dt, cot, a->partial.d(), planes, a->mbar.d(), a->cf.d(), t > 0);
             NBH_HIP(hipGetLastError());
         }
+        NBH_HIP(hipEventRecord(a->ev_free[b], c->stream));
     }
     // the initial forces (a0, j0) = F(x0, v0, m)
     double* x0 = a->ckpt.d();
