@@ -118,10 +118,12 @@ __global__ __launch_bounds__(kVB) void k_force_vjp(VjpArgs a)
             const double Bd = fma(Bx, dx, fma(By, dy, Bz * dz));
             const double p3 = -3.0 * p;
             const double c1 = fma(15.0 * du * Bd, ps, p3 * (Ad + Bu));
-            gx = fma(q, Ax, fma(c1, dx, fma(p3, fma(Bd, ux, du * Bx), gx)));
-            gy = fma(q, Ay, fma(c1, dy, fma(p3, fma(Bd, uy, du * By), gy)));
-            gz = fma(q, Az, fma(c1, dz, fma(p3, fma(Bd, uz, du * Bz), gz)));
+            // -3p ((B.d) u + (d.u) B) as e u + f B
             const double e = p3 * Bd;
+            const double f = p3 * du;
+            gx = fma(q, Ax, fma(c1, dx, fma(e, ux, fma(f, Bx, gx))));
+            gy = fma(q, Ay, fma(c1, dy, fma(e, uy, fma(f, By, gy))));
+            gz = fma(q, Az, fma(c1, dz, fma(e, uz, fma(f, Bz, gz))));
             hx = fma(q, Bx, fma(e, dx, hx));
             hy = fma(q, By, fma(e, dy, hy));
             hz = fma(q, Bz, fma(e, dz, hz));
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(kVB) void k_force_vjp(VjpArgs a)
             const double ad = fma(b.ax, dx, fma(b.ay, dy, b.az * dz));
             const double bu = fma(b.jx, ux, fma(b.jy, uy, b.jz * uz));
             const double bd = fma(b.jx, dx, fma(b.jy, dy, b.jz * dz));
-            const double gmo = fma(3.0 * du * p, bd, -q * (ad + bu));
+            const double gmo = fma(-f, bd, -q * (ad + bu));   // 3 (d.u) p (beta_o.d) - q (...)
             gm += MASK ? ((jt + q0) == k ? 0.0 : gmo) : gmo;
         }
     }
