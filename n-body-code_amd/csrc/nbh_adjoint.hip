@@ -576,6 +576,9 @@ int nbh_adj_forward(nbh_adjoint* a, const double* state0, double dt, int nsteps,
     nbh_ctx* c = a->ctx;
     if (c->cfg.precision != NBH_FP64) return fail(NBH_ERR_ARG, "the adjoint needs an FP64 context");
     NBH_HIP(hipSetDevice(c->cfg.device));
+    // a backward sweep that failed part-way may have left recompute work on
+    // its stream, which reads the checkpoints and writes the working state
+    NBH_HIP(hipStreamSynchronize(a->rstream));
     const int n = a->n;
     const int ncp = nsteps / a->every + 1;
     if (a->ckpt.reserve(sz(n, 13 * ncp)) != hipSuccess) return fail(NBH_ERR_ALLOC, "checkpoint allocation failed");
